@@ -1,0 +1,181 @@
+"""Field files and run configuration in the reference's formats.
+
+Field files (``u1_/u2_/u3_/p_NNNNxNNNNxNNNN_tTTTTT.dat``) are Fortran sequential-unformatted files:
+
+* record 1: ``t, Re, alp, bet, mpgx`` (f64) then ``nband, iter, dummint(88)`` (i32)
+  -- written at FOU3D.f90:2069, read at start.f90:1576;
+* record 2: ``N(4, 0:nband+1)`` (i32, column-major) -- FOU3D.f90:2070, start.f90:1598;
+* record 3: ``y, ddthetavi, dthdy`` (f64) -- FOU3D.f90:2071;
+* one record per plane: ``j, id, nx, nz`` (i32), ``y_j`` (f64), ``buff(nx, nz)`` (f64) with
+  ``nx = Nx+2`` interleaved (Re, Im) pairs for kx = 0..Nx/2 and ``nz = Nz`` z-modes in the reference's
+  column order (kz = 0..Nz/2-1, then -Nz/2..-1) -- FOU3D.f90:2077, u_to_buff FOU3D.f90:2266-2293.
+
+The plane buffer's (k, i) order is the reference's global column order (start.f90:963-981), so a file
+plane is one y-row of a column field.  Column fields here are ``complex128[ncol, nrows]`` with
+``ncol = (Nx/2+1)*Nz`` and rows ordered as the reference's ``x(j, column)`` (j fastest):
+u1/u3 rows j=0..nyu+1, u2 rows j=0..nyv+1, p rows j=1..nyu.
+"""
+from __future__ import annotations
+
+import dataclasses
+import re
+import struct
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class FieldFile:
+    t: float
+    Re: float
+    alp: float
+    bet: float
+    mpgx: float
+    nband: int
+    iter: int
+    N: np.ndarray          # int32 [4, nband+2] (Fortran N(4,0:nband+1))
+    y: np.ndarray
+    ddthetavi: float
+    dthdy: np.ndarray
+    j: np.ndarray          # plane indices as stored
+    planes: np.ndarray     # complex128 [nplanes, Nz, Kx]
+    plane_y: np.ndarray
+
+
+def _records(buf: bytes):
+    off = 0
+    n = len(buf)
+    while off < n:
+        (ln,) = struct.unpack_from("<i", buf, off)
+        start = off + 4
+        end = start + ln
+        (ln2,) = struct.unpack_from("<i", buf, end)
+        if ln2 != ln:
+            raise ValueError("corrupt Fortran record at byte %d" % off)
+        yield buf[start:end]
+        off = end + 4
+
+
+def read_field(path: str) -> FieldFile:
+    """Read one reference field file (start.f90:1545-1997 reads the same records)."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    recs = list(_records(buf))
+    if len(recs) < 4:
+        raise ValueError("%s: too few records" % path)
+    t, Re, alp, bet, mpgx = struct.unpack_from("<5d", recs[0], 0)
+    nband, it = struct.unpack_from("<2i", recs[0], 40)
+    Narr = np.frombuffer(recs[1], dtype="<i4")
+    N = Narr.reshape(-1, 4).T.copy()  # N(4, 0:nband+1), column-major
+    r3 = np.frombuffer(recs[2], dtype="<f8")
+    ny_pts = (len(r3) - 1) // 2
+    y = r3[:ny_pts].copy()
+    ddthetavi = float(r3[ny_pts])
+    dthdy = r3[ny_pts + 1:].copy()
+    js, ys, planes = [], [], []
+    for r in recs[3:]:
+        j, _id, nx, nz = struct.unpack_from("<4i", r, 0)
+        (yj,) = struct.unpack_from("<d", r, 16)
+        b = np.frombuffer(r, dtype="<f8", offset=24, count=nx * nz).reshape(nz, nx)
+        js.append(j)
+        ys.append(yj)
+        planes.append(b[:, 0::2] + 1j * b[:, 1::2])
+    return FieldFile(t, Re, alp, bet, mpgx, nband, it, N, y, ddthetavi, dthdy,
+                     np.array(js), np.array(planes), np.array(ys))
+
+
+def write_field(path: str, ff: FieldFile, dummint: np.ndarray | None = None) -> None:
+    """Write a field file with the record layout of record_out (FOU3D.f90:2063-2161)."""
+    def rec(payload: bytes) -> bytes:
+        return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+    if dummint is None:
+        dummint = np.zeros(88, dtype="<i4")
+    out = [rec(struct.pack("<5d2i", ff.t, ff.Re, ff.alp, ff.bet, ff.mpgx, ff.nband, ff.iter)
+               + np.asarray(dummint, "<i4").tobytes())]
+    out.append(rec(np.asarray(ff.N, "<i4").T.tobytes()))
+    out.append(rec(np.asarray(ff.y, "<f8").tobytes() + struct.pack("<d", ff.ddthetavi)
+                   + np.asarray(ff.dthdy, "<f8").tobytes()))
+    grid_id = 1
+    for jj, yj, pl in zip(ff.j, ff.plane_y, ff.planes):
+        nz, kx = pl.shape
+        b = np.empty((nz, 2 * kx), dtype="<f8")
+        b[:, 0::2] = pl.real
+        b[:, 1::2] = pl.imag
+        out.append(rec(struct.pack("<4id", int(jj), grid_id, 2 * kx, nz, float(yj)) + b.tobytes()))
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def planes_to_columns(planes: np.ndarray) -> np.ndarray:
+    """[nplanes, Nz, Kx] -> column field [ncol, nplanes] (column = k*Kx + i)."""
+    npl, nz, kx = planes.shape
+    return np.ascontiguousarray(planes.reshape(npl, nz * kx).T)
+
+
+def columns_to_planes(cols: np.ndarray, Nz: int) -> np.ndarray:
+    ncol, nrows = cols.shape
+    return np.ascontiguousarray(cols.T.reshape(nrows, Nz, ncol // Nz))
+
+
+def load_state(prefix_dir: str, tag: str, ny: int):
+    """Load u1, u2, u3, p of one snapshot ``<dir>/{u1,u2,u3,p}<tag>`` as reference column fields.
+
+    The rows are placed by the plane index j stored in each record, so files written with the
+    legacy multi-band header read the same way (start.f90:1611-1620 only uses N(:,0) and N(:,3)).
+    Returns (u1, u2, u3, p, header FieldFile of u1).
+    """
+    nyv = ny - 2
+    nyu = nyv + 1
+    rows = {"u1": (0, nyu + 2), "u3": (0, nyu + 2), "u2": (0, nyv + 2), "p": (1, nyu)}
+    out = {}
+    head = None
+    for name, (j0, nr) in rows.items():
+        ff = read_field("%s/%s%s" % (prefix_dir, name, tag))
+        if head is None or name == "u1":
+            head = ff if name == "u1" else head
+        nz, kx = ff.planes.shape[1:]
+        cols = np.zeros((nz * kx, nr), dtype=np.complex128)
+        for jj, pl in zip(ff.j, ff.planes):
+            r = int(jj) - j0
+            if 0 <= r < nr:
+                cols[:, r] = pl.reshape(-1)
+        out[name] = cols
+    return out["u1"], out["u2"], out["u3"], out["p"], head
+
+
+# ---------------------------------------------------------------------------------------------
+# input.in : positional text file read with fixed formats (start.f90:86-252, FORMATs :485-488)
+
+_INPUT_KEYS = ["Re", "alp", "bet", "Ngalx1", "Ngalx2", "Ngalz1", "Ngalz2", "ny", "dyq", "pp",
+               "flgin", "fnmeb", "dirin", "dirout", "CFL", "nwrite", "flpress", "mpgx", "maxerr",
+               "maxt", "maxA", "type"]
+
+
+def read_input(path: str) -> dict:
+    """Parse the reference's ``input.in`` by key (the reference reads it by line position; the keys
+    of the shipped files are unique, so reading by key gives the same values)."""
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"^([A-Za-z][A-Za-z0-9_]*)\s*=\s*(\S+)", line)
+            if not m:
+                continue
+            key, v = m.group(1), m.group(2)
+            if key in ("fnmeb", "dirin", "dirout", "dirlist", "heading"):
+                vals[key] = v
+                continue
+            try:
+                vals[key] = float(v.replace("d", "e").replace("D", "e"))
+            except ValueError:
+                vals[key] = v
+    cfg = {
+        "Re": vals["Re"], "alp": vals["alp"], "bet": vals["bet"],
+        "Mx": int(vals["Ngalx1"]), "Mz": int(vals["Ngalz1"]), "ny": int(vals["ny"]),
+        "dyq": vals["dyq"], "ppp": int(vals["pp"]), "CFL": vals["CFL"],
+        "flag_ctpress": int(vals.get("flpress", 1)), "mpgx": vals.get("mpgx", 0.0),
+        "flag_init": int(vals.get("flgin", 1)), "fnmeb": vals.get("fnmeb", ""),
+        "dirin": vals.get("dirin", "./input/"), "dirout": vals.get("dirout", "./output/"),
+        "nwrite": int(vals.get("nwrite", 1)), "maxt": vals.get("maxt", 1e30),
+    }
+    return cfg

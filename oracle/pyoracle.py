@@ -1,0 +1,265 @@
+"""ORACLE TEST INFRASTRUCTURE -- imported only by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py.  Never imported by the product.
+
+* ``Oracle``: ctypes wrapper of liboracle.so (dns_oracle.c, the C restatement of the reference's
+  RK3 step).
+* ``run_reference``: runs the reference's own Fortran hot-path code (oracle/_ref/ref_harness, built by
+  ``make -C oracle ref`` from the reference sources) on one state and returns its outputs.
+
+Column fields are complex128 arrays [ncol, nrows] in the reference's x(j, column) order.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def build(ref: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [ctypes.c_int] * 2
+        for name in ("orc_destroy", "orc_meanflow_ctP"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        for name in ("orc_rhs0", "orc_nonlinear", "orc_solveU", "orc_meanflow_ctU", "orc_solveP",
+                     "orc_vcorr", "orc_set_threads"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_double]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_set_nu.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_nu.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_du.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_flux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_psi.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_get_scalars.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_get_setup.argtypes = [ctypes.c_void_p] + [_dp] * 8
+        L.orc_plane_fft.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    def __init__(self, Mx, Mz, ny, Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0,
+                 ppp=5, flag_ctpress=1, threads=None):
+        L = lib()
+        self.h = L.orc_create(Mx, Mz, ny, Re, alp, bet, CFL, mpgx, dyq, ppp, flag_ctpress)
+        if not self.h:
+            raise ValueError("unsupported sizes %d x %d" % (Mx, Mz))
+        s = (ctypes.c_int * 8)()
+        L.orc_sizes(self.h, s)
+        self.Nx, self.Nz, self.Kx, self.C, self.nyu, self.nyv = s[0], s[1], s[2], s[3], s[4], s[5]
+        self.Mx, self.Mz, self.ny = Mx, Mz, ny
+        if threads:
+            L.orc_set_threads(self.h, threads)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def shapes(self):
+        C, ny = self.C, self.ny
+        return {"u1": (C, ny + 1), "u2": (C, ny), "u3": (C, ny + 1), "p": (C, ny - 1),
+                "Nu1": (C, ny - 1), "Nu2": (C, ny - 2), "Nu3": (C, ny - 1)}
+
+    def set_state(self, u1, u2, u3, p, t=0.0, it=0, QxT=0.0):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (u1, u2, u3, p)]
+        lib().orc_set_state(self.h, *[_ptr(a.view(np.float64)) for a in arrs], t, it, QxT)
+
+    def get_state(self):
+        sh = self.shapes()
+        out = [np.zeros(sh[k], np.complex128) for k in ("u1", "u2", "u3", "p")]
+        lib().orc_get_state(self.h, *[_ptr(a.view(np.float64)) for a in out])
+        return out
+
+    def set_nu(self, Nu1, Nu2, Nu3):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (Nu1, Nu2, Nu3)]
+        lib().orc_set_nu(self.h, *[_ptr(a.view(np.float64)) for a in arrs])
+
+    def get_nu(self):
+        sh = self.shapes()
+        out = [np.zeros(sh[k], np.complex128) for k in ("Nu1", "Nu2", "Nu3")]
+        lib().orc_get_nu(self.h, *[_ptr(a.view(np.float64)) for a in out])
+        return out
+
+    def get_du(self):
+        sh = self.shapes()
+        out = [np.zeros(sh[k], np.complex128) for k in ("Nu1", "Nu2", "Nu3")]
+        lib().orc_get_du(self.h, *[_ptr(a.view(np.float64)) for a in out])
+        return out
+
+    def get_flux(self):
+        C, ny = self.C, self.ny
+        out = [np.zeros((C, ny), np.complex128), np.zeros((C, ny + 1), np.complex128),
+               np.zeros((C, ny), np.complex128)]
+        lib().orc_get_flux(self.h, *[_ptr(a.view(np.float64)) for a in out])
+        return out
+
+    def get_psi(self):
+        out = [np.zeros((self.C, self.ny - 1), np.complex128) for _ in range(2)]
+        lib().orc_get_psi(self.h, *[_ptr(a.view(np.float64)) for a in out])
+        return out
+
+    def scalars(self):
+        sc = np.zeros(8)
+        lib().orc_get_scalars(self.h, _ptr(sc))
+        return dict(zip(["t", "dt", "dti", "mpgx", "Qx", "dgx", "dtc", "QxT"], sc))
+
+    def set_dt(self, dt):
+        lib().orc_set_dt(self.h, dt)
+
+    def setup(self):
+        ny, C = self.ny, self.C
+        yu, yv = np.zeros(ny + 1), np.zeros(ny)
+        du, dv = np.zeros(ny + 1), np.zeros(ny)
+        d2u, d2v = np.zeros((ny + 1, 3)), np.zeros((ny, 3))
+        DG = np.zeros((C, ny - 1, 3))
+        w = np.zeros(6)
+        lib().orc_get_setup(self.h, *[_ptr(a) for a in (yu, yv, du, dv, d2u, d2v, DG, w)])
+        return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv, dyu2i=d2u, dyv2i=d2v, DG=DG, dtv=w[0],
+                    gw=w[1:3], gbc=w[3], gi=w[4:6])
+
+    def rhs0(self, k): lib().orc_rhs0(self.h, k)
+    def nonlinear(self, k): lib().orc_nonlinear(self.h, k)
+    def solveU(self, k): lib().orc_solveU(self.h, k)
+    def meanflow_ctU(self, k): lib().orc_meanflow_ctU(self.h, k)
+    def solveP(self, k): lib().orc_solveP(self.h, k)
+    def vcorr(self, k): lib().orc_vcorr(self.h, k)
+
+    def step(self, n=1):
+        hist = np.zeros((n, 4))
+        lib().orc_step(self.h, n, _ptr(hist))
+        return hist
+
+
+def plane_fft(Mx, Mz, planes):
+    """four_to_phys_du then phys_to_four_du on real (Mx+2) x Mz planes ([n, Mz, Mx+2] float64)."""
+    planes = np.ascontiguousarray(planes, dtype=np.float64)
+    n = planes.shape[0]
+    out = np.zeros((2 * n, Mz, Mx + 2))
+    lib().orc_plane_fft(Mx, Mz, n, _ptr(planes), _ptr(out))
+    return out[:n], out[n:]
+
+
+# ------------------------------------------------------------------------------------------------
+def have_reference() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=0.0, it=0,
+                  Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
+                  flag_ctpress=1, QxT=0.0, Nu=None, fft_planes=None, timeout=3600):
+    """Run the reference's compiled hot path (oracle/_ref/ref_harness) on one state.
+
+    mode 1: ``nsteps`` full RK3 steps -> dict(u1,u2,u3,p,Nu1,Nu2,Nu3,t,dt,mpgx,Qx,dgx,hist)
+    mode 2: one substep ``kRK`` with the stage-by-stage arrays (``dt`` is used when kRK > 1)
+    mode 3: ``fft_planes`` [n, Mz, Mx+2] through four_to_phys_du / phys_to_four_du
+    """
+    if not have_reference():
+        raise FileNotFoundError(REF_HARNESS)
+    nsteps_field = nsteps if mode != 3 else fft_planes.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "params.txt"), "w") as f:
+            f.write("%d\n%d %d %d\n" % (mode, Mx, Mz, ny))
+            f.write("%.17e %.17e %.17e %.17e %.17e\n" % (Re, alp, bet, CFL, mpgx))
+            f.write("%.17e %d %d\n" % (dyq, ppp, flag_ctpress))
+            f.write("%.17e %d %d %d %.17e %.17e\n" % (t, it, nsteps_field, kRK, dt, QxT))
+        if mode == 3:
+            np.ascontiguousarray(fft_planes, dtype=np.float64).tofile(os.path.join(d, "fft_in.bin"))
+        else:
+            with open(os.path.join(d, "state_in.bin"), "wb") as f:
+                for a in (u1, u2, u3, p):
+                    f.write(np.ascontiguousarray(a, dtype=np.complex128).tobytes())
+                if mode == 2:
+                    for a in Nu:
+                        f.write(np.ascontiguousarray(a, dtype=np.complex128).tobytes())
+        env = dict(os.environ)
+        env.setdefault("OMP_NUM_THREADS", "1")
+        subprocess.run([REF_HARNESS], cwd=d, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE, timeout=timeout, env=env)
+        if mode == 3:
+            out = np.fromfile(os.path.join(d, "fft_out.bin"), dtype=np.float64)
+            n = fft_planes.shape[0]
+            out = out.reshape(2 * n, Mz, Mx + 2)
+            return out[:n], out[n:]
+        Nx, Nz = (2 * Mx) // 3, (2 * Mz) // 3
+        C = (Nx // 2 + 1) * Nz
+        nyv = ny - 2
+        nyu = nyv + 1
+        sizes = [("u1", nyu + 2), ("u2", nyv + 2), ("u3", nyu + 2), ("p", nyu), ("Nu1", nyu),
+                 ("Nu2", nyv), ("Nu3", nyu)]
+        raw = np.fromfile(os.path.join(d, "state_out.bin"), dtype=np.uint8)
+        res = {}
+        off = 0
+        for name, nr in sizes:
+            nbytes = C * nr * 16
+            res[name] = raw[off:off + nbytes].view(np.complex128).reshape(C, nr).copy()
+            off += nbytes
+        sc = raw[off:off + 40].view(np.float64)
+        res.update(dict(t=sc[0], dt=sc[1], mpgx=sc[2], Qx=sc[3], dgx=sc[4]))
+        if mode == 1:
+            res["hist"] = np.loadtxt(os.path.join(d, "hist.txt"), ndmin=2)
+        if mode == 2:
+            st = np.fromfile(os.path.join(d, "stages.bin"), dtype=np.uint8)
+            off = 0
+
+            def take(nr):
+                nonlocal off
+                nbytes = C * nr * 16
+                a = st[off:off + nbytes].view(np.complex128).reshape(C, nr).copy()
+                off += nbytes
+                return a
+            res["rhs0"] = [take(nyu), take(nyv), take(nyu)]
+            res["Nu_new"] = [take(nyu), take(nyv), take(nyu)]
+            res["du_nl"] = [take(nyu), take(nyv), take(nyu)]
+            res["flux"] = [take(nyv + 2), take(nyu + 2), take(nyv + 2)]
+            res["u_solved"] = [take(nyu + 2), take(nyv + 2), take(nyu + 2)]
+            res["u1_meanflow"] = take(nyu + 2)
+            res["p_new"], res["psi"] = take(nyu), take(nyu)
+            res["u_corr"] = [take(nyu + 2), take(nyv + 2), take(nyu + 2)]
+            res["div"] = take(nyu)
+            sc2 = st[off:off + 40].view(np.float64)
+            res["stage_scalars"] = dict(dt=sc2[0], t=sc2[1], mpgx=sc2[2], dgx=sc2[3], dtc=sc2[4])
+        sb = np.fromfile(os.path.join(d, "setup_out.bin"), dtype=np.float64)
+        o = 0
+        setup = {}
+        for name, n in (("yu", ny + 1), ("yv", ny), ("dthdyu", ny + 1), ("dthdyv", ny),
+                        ("dyu2i", 3 * (ny + 1)), ("dyv2i", 3 * ny), ("DG", 3 * (ny - 1) * C)):
+            setup[name] = sb[o:o + n].copy()
+            o += n
+        setup["dtv"] = sb[o]
+        setup["gw"] = sb[o + 1:o + 3]
+        setup["gi"] = sb[o + 3:o + 5]
+        setup["gbc"] = sb[o + 5]
+        res["setup"] = setup
+        return res
