@@ -1,0 +1,132 @@
+"""Benchmark: RK3 steps/s of the channel-flow DNS time step at 384 x 384 x 257 (576 x 576 dealiased
+physical points) on one MI355X, FP64 and mixed precision, with the HBM roofline of the dominant kernel
+and the reference's own CPU code timed on the host as a baseline.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+
+Prints one JSON line.  value = FP64 steps/s (the fields, the RK3 accumulation and the solves all FP64);
+the mixed-precision run (FP32 plane stage) is reported under "mixed" with its relative L2 velocity
+error against the FP64 run after the same number of steps from the same initial state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "final-single-block-_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "RK3 steps/s at 384x384x257: FP64 vs mixed-prec at <=1e-6 err; % HBM peak"
+MX, MZ, NY = 576, 576, 257            # 384 x 384 Fourier modes x 257 y points
+RE, RETAU = 10000.0, 550.0            # Re_tau = Re sqrt(-mpgx) under a constant pressure gradient
+MPGX = -(RETAU / RE) ** 2
+KERNELS = ["zinv", "xpass", "zfwd", "col_up", "col_down", "col_final"]
+
+
+def gpu_leg(st, prec, steps, warmup):
+    import dnship
+    g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX, precision=prec)
+    g.upload(*st)
+    g.step(warmup)
+    g.sync()
+    g.set_timing(1)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    tim = g.timing()
+    sc = g.scalars()
+    fields = g.download()
+    g.close()
+    return (t1 - t0) / steps, tim, sc, fields
+
+
+def cpu_leg(st):
+    """The reference's own hot path (oracle/_ref, compiled from its sources) on one host core, timed on a
+    bounded sample: one RK3 substep of the same 384x384x257 workload; falls back to the oracle's C
+    restatement on all host cores when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if pyoracle.have_reference():
+        t = pyoracle.time_reference_substeps(MX, MZ, NY, *st, nsub=1, Re=RE, mpgx=MPGX)
+        return {"value": 1.0 / (3.0 * t), "unit": "steps/s", "cores": 1, "kind": "reference",
+                "sample": "one RK3 substep (kRK=1) of the 384x384x257 workload, single MPI rank, %.1f s; "
+                          "steps/s = 1/(3 x substep time)" % t}
+    n = len(os.sched_getaffinity(0))
+    o = pyoracle.Oracle(MX, MZ, NY, Re=RE, mpgx=MPGX, threads=n)
+    o.set_state(*st)
+    t0 = time.perf_counter()
+    o.step(1)
+    t = time.perf_counter() - t0
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": n, "kind": "port",
+            "sample": "one RK3 step of the 384x384x257 workload, OpenMP over %d threads, %.1f s" % (n, t)}
+
+
+def traffic(kernel, prec_tag):
+    path = os.path.join(ROOT, "profiles", "traffic_%s_384x384x257_%s.json" % (kernel, prec_tag))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import dnship
+    import dns_ic
+    import dns_roofline
+
+    st = dns_ic.seeded_channel(MX, MZ, NY, seed=3, amp=0.1, broadband=True)
+    sec64, tim64, sc64, f64 = gpu_leg(st, dnship.PREC_FP64, args.steps, args.warmup)
+    out = {"metric": METRIC, "value": 1.0 / sec64, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": sec64 * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: laminar Poiseuille + seeded (seed 3) broadband divergence-free perturbation, "
+                   "amplitude 0.1",
+           "config": {"workload": "RK3 step, 384x384 Fourier modes x 257 y points (576x576 dealiased), "
+                                  "box 2pi x 2 x pi, Re=10000, mpgx=-(550/Re)^2 (Re_tau~550), CFL=0.7",
+                      "Mx": MX, "Mz": MZ, "ny": NY, "precision": "fp64"}}
+    kms = {k: tim64[k] for k in KERNELS}
+    dom = max(KERNELS, key=lambda k: kms[k])
+    roof = dns_roofline.roofline(dom, MX, MZ, NY, False, kms[dom])
+    roof["kernel"] = dom
+    roof["avg_ms"] = kms[dom]
+    roof["traffic"] = traffic(dom, "fp64")
+    out["roofline"] = roof
+    by = dns_roofline.kernel_bytes(MX, MZ, NY, False)
+    out["kernels"] = {k: {"avg_ms": kms[k], "GBs": by[k] / (kms[k] * 1e-3) / 1e9,
+                          "frac_hbm": by[k] / (kms[k] * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS}
+                      for k in KERNELS}
+    out["step_frac_hbm"] = by["step"] / sec64 / 1e9 / dns_roofline.HBM_PEAK_GBS
+    out["max_div"] = sc64["divmax"]
+    if not args.no_mixed:
+        secmx, timmx, scmx, fmx = gpu_leg(st, dnship.PREC_MIXED, args.steps, args.warmup)
+        err = {k: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for k, a, b in zip(("u1", "u2", "u3"), fmx, f64)}
+        bym = dns_roofline.kernel_bytes(MX, MZ, NY, True)
+        out["mixed"] = {"value": 1.0 / secmx, "ms_per_step": secmx * 1e3, "speedup_vs_fp64": sec64 / secmx,
+                        "rel_l2_vs_fp64": err, "steps_compared": args.steps + args.warmup,
+                        "kernels_ms": {k: timmx[k] for k in KERNELS},
+                        "step_frac_hbm": bym["step"] / secmx / 1e9 / dns_roofline.HBM_PEAK_GBS}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_leg(st)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

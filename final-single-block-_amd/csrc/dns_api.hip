@@ -1,0 +1,715 @@
+// Host side of the C-ABI (include/dns_hip.h): set-up restated from start.f90 for one rank, device
+// memory, and the launch sequences of the staged and fused RK3 substeps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dns_hip.h"
+#include "dns_kernels.h"
+
+using dnsk::ColArgs;
+using dnsk::DnsState;
+using dnsk::PlaneArgs;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                       \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(DNS_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char *dns_last_error(void) { return g_err.c_str(); }
+
+struct dns_handle {
+  dns_params prm;
+  int Mx, Mz, Nx, Nz, Kx, C, ny, nyu, nyv;
+  bool fp32;
+  int batch;  // planes per plane-stage group
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  bool uploaded = false, timing = false;
+  // host set-up (start.f90)
+  std::vector<double> yu, yv, dthdyu, dthdyv, dyu2i, dyv2i;
+  double dtv, gw1, gw2, gbc, gi1, gi2;
+  double aRK[3], bRK[3], gRK[3], cRK[3], dRK[3];
+  // device
+  double2 *u1, *u2, *u3, *p, *psi, *div, *du1, *du2, *du3, *Nu1, *Nu2, *Nu3;
+  double2 *Nxz1, *Nxz2, *Nxz3, *uvf, *vvc, *wvf;
+  double *w3u, *w3v, *cP, *mscr;
+  double *d_yu, *d_yv, *d_dthdyu, *d_dthdyv, *d_dyu2i, *d_dyv2i;
+  void *I, *P, *twz, *twx, *twh;
+  double2 *twz64, *twx64, *twh64;  // FP64 twiddles for the reference-format plane utilities
+  double2 *stage;                  // layout conversion buffer
+  DnsState *st;
+  std::vector<void *> allocs;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev[8];
+  double tsum[7];
+  int tcount;
+};
+
+// ------------------------------------------------------------------------------------------------
+// set-up (restates start.f90 for one rank)
+static double powi(double a, int b) {  // x**n with an integer exponent, by binary exponentiation
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return r;
+}
+
+// ygrid : start.f90:665-806
+static void host_grid(dns_handle *h) {
+  const int nn = h->nyv + 1, ppp = h->prm.ppp;
+  const double dyq = h->prm.dyq;
+  const double qqq = nn / 2.0;
+  const double bbb = (dyq - 1.0) /
+      (qqq * (1.0 - powi(qqq, ppp - 1) + dyq * ((1.0 - qqq) * powi(qqq, ppp - 1) - powi(1.0 - qqq, ppp))));
+  const double aaa = (1.0 - bbb * powi(qqq, ppp)) / qqq;
+  const int ny = h->ny;
+  h->yu.assign(ny + 1, 0.0);
+  h->yv.assign(ny, 0.0);
+  h->dthdyu.assign(ny + 1, 0.0);
+  h->dthdyv.assign(ny, 0.0);
+  h->dyu2i.assign(3 * (ny + 1), 0.0);
+  h->dyv2i.assign(3 * ny, 0.0);
+  std::vector<double> &yu = h->yu, &yv = h->yv;
+  yu[0] = aaa * (-1.0 + .5 - qqq) + bbb * powi(-1.0 + .5 - qqq, ppp);
+  for (int j = 0; j <= nn; j++) {
+    yv[j] = aaa * (j - qqq) + bbb * powi(j - qqq, ppp);
+    yu[j + 1] = aaa * (j + .5 - qqq) + bbb * powi(j + .5 - qqq, ppp);
+  }
+  h->dthdyu[0] = 1.0 / (aaa + ppp * bbb * powi(-1.0 + .5 - qqq, ppp - 1));
+  for (int j = 0; j <= nn; j++) {
+    h->dthdyv[j] = 1.0 / (aaa + ppp * bbb * powi(j - qqq, ppp - 1));
+    h->dthdyu[j + 1] = 1.0 / (aaa + ppp * bbb * powi(j + .5 - qqq, ppp - 1));
+  }
+  auto U = [&](int c, int j) -> double & { return h->dyu2i[3 * j + c - 1]; };
+  auto V = [&](int c, int j) -> double & { return h->dyv2i[3 * j + c - 1]; };
+  auto sq = [](double x) { return x * x; };
+  int j = 0;
+  U(1, j) = 0.0;
+  U(2, j) = -2.0 / sq(yu[j + 1] - yu[j]);
+  U(3, j) = 1.0 / sq(yu[j + 1] - yu[j]);
+  V(1, j) = 0.0;
+  V(2, j) = -2.0 / sq(yv[j + 1] - yv[j]);
+  V(3, j) = 1.0 / sq(yv[j + 1] - yv[j]);
+  for (j = 1; j <= nn - 1; j++) {
+    U(1, j) = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+    U(2, j) = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+    U(3, j) = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+    V(1, j) = 2.0 / ((yv[j] - yv[j - 1]) * (yv[j + 1] - yv[j - 1]));
+    V(2, j) = -2.0 / ((yv[j + 1] - yv[j]) * (yv[j] - yv[j - 1]));
+    V(3, j) = 2.0 / ((yv[j + 1] - yv[j]) * (yv[j + 1] - yv[j - 1]));
+  }
+  j = nn;
+  U(1, j) = 2.0 / ((yu[j] - yu[j - 1]) * (yu[j + 1] - yu[j - 1]));
+  U(2, j) = -2.0 / ((yu[j + 1] - yu[j]) * (yu[j] - yu[j - 1]));
+  U(3, j) = 2.0 / ((yu[j + 1] - yu[j]) * (yu[j + 1] - yu[j - 1]));
+  V(1, j) = 1.0 / sq(yv[j] - yv[j - 1]);
+  V(2, j) = -2.0 / sq(yv[j] - yv[j - 1]);
+  V(3, j) = 0.0;
+  j = nn + 1;
+  U(1, j) = 1.0 / sq(yu[j] - yu[j - 1]);
+  U(2, j) = -2.0 / sq(yu[j] - yu[j - 1]);
+  U(3, j) = 0.0;
+  // dtv (start.f90:476) with k2F_x(Nx/2) and k2F_z(Nz/2) of def_k (start.f90:838-862)
+  const double k2x = -((h->prm.alp * (h->Nx / 2)) * (h->prm.alp * (h->Nx / 2)));
+  const double k2z = -((h->prm.bet * (h->Nz / 2 - 1)) * (h->prm.bet * (h->Nz / 2 - 1)));
+  h->dtv = h->prm.Re / (-k2x - k2z + 4.0 / sq(yv[1] - yv[0]));
+  // wall weights (start.f90:518-537)
+  const int nyu = h->nyu, nyv = h->nyv;
+  h->gw1 = -(yv[0] - yu[0]) / (yv[0] - yu[1]);
+  h->gw2 = (yu[nyu + 1] - yv[nyv + 1]) / (yv[nyv + 1] - yu[nyu]);
+  h->gbc = (yu[0] - yv[0]) / (yu[1] - yv[0]);
+  h->gi1 = (yu[0] - yu[1]) / (yv[0] - yu[1]);
+  h->gi2 = (yu[nyu + 1] - yu[nyu]) / (yv[nyv + 1] - yu[nyu]);
+  // Runge-Kutta coefficients (Le & Moin 1991), start.f90:439-453
+  const double a[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0}, g[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double c[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0}, d[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+  for (int q = 0; q < 3; q++) {
+    h->aRK[q] = a[q]; h->bRK[q] = a[q]; h->gRK[q] = g[q]; h->cRK[q] = c[q]; h->dRK[q] = d[q];
+  }
+}
+
+template <typename T>
+static int dalloc(dns_handle *h, T **p, size_t n) {
+  void *q = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&q, n * sizeof(T)));
+  HIPCHK(hipMemset(q, 0, n * sizeof(T)));
+  h->allocs.push_back(q);
+  h->bytes += n * sizeof(T);
+  *p = reinterpret_cast<T *>(q);
+  return 0;
+}
+
+static bool fft_len_ok(int n) {
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  return n > 0 && m == 1;
+}
+
+template <typename R>
+static std::vector<R> twiddles(int n) {  // e^{-2 pi i m/n} as (re, im) pairs
+  std::vector<R> t(2 * (size_t)n);
+  for (int m = 0; m < n; m++) {
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    t[2 * m] = (R)cosl(a);
+    t[2 * m + 1] = (R)(-sinl(a));
+  }
+  return t;
+}
+
+static void destroy_all(dns_handle *h) {
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) {
+    for (int q = 0; q < 8; q++) (void)hipEventDestroy(h->ev[q]);
+    (void)hipStreamDestroy(h->stream);
+  }
+  delete h;
+}
+
+extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
+  if (!prm || !out) return fail(DNS_ERR_ARG, "dns_create: null argument");
+  *out = nullptr;
+  const int Mx = prm->Mx, Mz = prm->Mz, ny = prm->ny;
+  if (Mx <= 0 || Mz <= 0 || Mx % 6 != 0 || Mz % 6 != 0)
+    return fail(DNS_ERR_ARG, "dns_create: Mx=%d, Mz=%d must be positive multiples of 6 (Nx = 2Mx/3 even)", Mx, Mz);
+  // FFTPACK factors 2, 3, 4 only; any other prime factor is fatal there (rft_buff.f:308-310)
+  if (!fft_len_ok(Mx) || !fft_len_ok(Mz))
+    return fail(DNS_ERR_ARG, "dns_create: Mx=%d, Mz=%d have a prime factor other than 2 and 3", Mx, Mz);
+  if (!dnsk::plane_size_supported(Mx) || !dnsk::plane_size_supported(Mz))
+    return fail(DNS_ERR_ARG, "dns_create: no kernel instantiated for Mx=%d, Mz=%d", Mx, Mz);
+  if (Mz % dnsk::xpass_rows_per_block(Mx) != 0)
+    return fail(DNS_ERR_ARG, "dns_create: Mz=%d must be a multiple of %d for Mx=%d", Mz, dnsk::xpass_rows_per_block(Mx), Mx);
+  if (ny < 6) return fail(DNS_ERR_ARG, "dns_create: ny=%d too small", ny);
+  if (prm->ppp < 1) return fail(DNS_ERR_ARG, "dns_create: pp=%d must be >= 1", prm->ppp);
+  if (!(prm->Re > 0) || !(prm->CFL > 0) || !(prm->alp > 0) || !(prm->bet > 0))
+    return fail(DNS_ERR_ARG, "dns_create: Re, CFL, alp, bet must be positive");
+  if (prm->flag_ctpress != 0 && prm->flag_ctpress != 1)
+    return fail(DNS_ERR_ARG, "dns_create: flag_ctpress=%d must be 0 or 1", prm->flag_ctpress);
+  if (prm->precision != DNS_PREC_FP64 && prm->precision != DNS_PREC_MIXED)
+    return fail(DNS_ERR_ARG, "dns_create: unknown precision %d", prm->precision);
+
+  dns_handle *h = new dns_handle();
+  h->prm = *prm;
+  h->Mx = Mx; h->Mz = Mz; h->ny = ny;
+  h->Nx = 2 * Mx / 3; h->Nz = 2 * Mz / 3;
+  h->Kx = h->Nx / 2 + 1; h->C = h->Kx * h->Nz;
+  h->nyv = ny - 2; h->nyu = h->nyv + 1;
+  h->fp32 = prm->precision == DNS_PREC_MIXED;
+  const int nplanes = 2 * h->nyu;
+  h->batch = (prm->plane_batch > 0 && prm->plane_batch < nplanes) ? prm->plane_batch : nplanes;
+  host_grid(h);
+
+  auto chk = [&](int rc) { return rc; };
+  int rc = 0;
+  const size_t C = h->C, ru = h->nyu + 2, rv = h->nyv + 2, rp = h->nyu;
+  rc |= chk(dalloc(h, &h->u1, C * ru)); rc |= chk(dalloc(h, &h->u3, C * ru)); rc |= chk(dalloc(h, &h->u2, C * rv));
+  rc |= chk(dalloc(h, &h->p, C * rp)); rc |= chk(dalloc(h, &h->psi, C * rp)); rc |= chk(dalloc(h, &h->div, C * rp));
+  rc |= chk(dalloc(h, &h->du1, C * ru)); rc |= chk(dalloc(h, &h->du3, C * ru)); rc |= chk(dalloc(h, &h->du2, C * rv));
+  rc |= chk(dalloc(h, &h->Nu1, C * rp)); rc |= chk(dalloc(h, &h->Nu3, C * rp)); rc |= chk(dalloc(h, &h->Nu2, C * rp));
+  rc |= chk(dalloc(h, &h->Nxz1, C * rp)); rc |= chk(dalloc(h, &h->Nxz3, C * rp)); rc |= chk(dalloc(h, &h->Nxz2, C * rp));
+  rc |= chk(dalloc(h, &h->uvf, C * rv)); rc |= chk(dalloc(h, &h->wvf, C * rv)); rc |= chk(dalloc(h, &h->vvc, C * ru));
+  rc |= chk(dalloc(h, &h->w3u, C * ru)); rc |= chk(dalloc(h, &h->w3v, C * rv)); rc |= chk(dalloc(h, &h->cP, C * rp));
+  rc |= chk(dalloc(h, &h->mscr, 2 * ru));
+  rc |= chk(dalloc(h, &h->d_yu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_yv, (size_t)ny));
+  rc |= chk(dalloc(h, &h->d_dthdyu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_dthdyv, (size_t)ny));
+  rc |= chk(dalloc(h, &h->d_dyu2i, 3 * ((size_t)ny + 1))); rc |= chk(dalloc(h, &h->d_dyv2i, 3 * (size_t)ny));
+  rc |= chk(dalloc(h, &h->stage, C * ru));
+  rc |= chk(dalloc(h, &h->st, 1));
+  const size_t rsz = h->fp32 ? sizeof(float2) : sizeof(double2);
+  const size_t iel = (size_t)h->batch * 3 * Mz * h->Kx, pel = (size_t)h->batch * 4 * Mz * h->Kx;
+  char *Ib = nullptr, *Pb = nullptr, *tz = nullptr, *tx = nullptr, *th = nullptr;
+  rc |= chk(dalloc(h, &Ib, iel * rsz)); rc |= chk(dalloc(h, &Pb, pel * rsz));
+  rc |= chk(dalloc(h, &tz, (size_t)Mz * rsz)); rc |= chk(dalloc(h, &tx, (size_t)Mx * rsz));
+  rc |= chk(dalloc(h, &th, (size_t)(Mx / 2) * rsz));
+  rc |= chk(dalloc(h, &h->twz64, (size_t)Mz)); rc |= chk(dalloc(h, &h->twx64, (size_t)Mx));
+  rc |= chk(dalloc(h, &h->twh64, (size_t)Mx / 2));
+  if (rc) {
+    std::string msg = g_err;
+    destroy_all(h);
+    return fail(DNS_ERR_HIP, "dns_create: device allocation failed: %s", msg.c_str());
+  }
+  h->I = Ib; h->P = Pb; h->twz = tz; h->twx = tx; h->twh = th;
+  auto up = [&](void *dst, const void *src, size_t n) { return hipMemcpy(dst, src, n, hipMemcpyHostToDevice); };
+  hipError_t e = hipSuccess;
+  e = e ? e : up(h->d_yu, h->yu.data(), h->yu.size() * 8);
+  e = e ? e : up(h->d_yv, h->yv.data(), h->yv.size() * 8);
+  e = e ? e : up(h->d_dthdyu, h->dthdyu.data(), h->dthdyu.size() * 8);
+  e = e ? e : up(h->d_dthdyv, h->dthdyv.data(), h->dthdyv.size() * 8);
+  e = e ? e : up(h->d_dyu2i, h->dyu2i.data(), h->dyu2i.size() * 8);
+  e = e ? e : up(h->d_dyv2i, h->dyv2i.data(), h->dyv2i.size() * 8);
+  if (h->fp32) {
+    auto a = twiddles<float>(Mz), b = twiddles<float>(Mx), c = twiddles<float>(Mx / 2);
+    e = e ? e : up(tz, a.data(), a.size() * 4);
+    e = e ? e : up(tx, b.data(), b.size() * 4);
+    e = e ? e : up(th, c.data(), c.size() * 4);
+  } else {
+    auto a = twiddles<double>(Mz), b = twiddles<double>(Mx), c = twiddles<double>(Mx / 2);
+    e = e ? e : up(tz, a.data(), a.size() * 8);
+    e = e ? e : up(tx, b.data(), b.size() * 8);
+    e = e ? e : up(th, c.data(), c.size() * 8);
+  }
+  {
+    auto a = twiddles<double>(Mz), b = twiddles<double>(Mx), c = twiddles<double>(Mx / 2);
+    e = e ? e : up(h->twz64, a.data(), a.size() * 8);
+    e = e ? e : up(h->twx64, b.data(), b.size() * 8);
+    e = e ? e : up(h->twh64, c.data(), c.size() * 8);
+  }
+  DnsState s0;
+  memset(&s0, 0, sizeof s0);
+  s0.mpgx = prm->mpgx;
+  e = e ? e : up(h->st, &s0, sizeof s0);
+  e = e ? e : hipStreamCreate(&h->stream);
+  for (int q = 0; q < 8 && e == hipSuccess; q++) e = hipEventCreate(&h->ev[q]);
+  if (e != hipSuccess) {
+    destroy_all(h);
+    return fail(DNS_ERR_HIP, "dns_create: %s", hipGetErrorString(e));
+  }
+  memset(h->tsum, 0, sizeof h->tsum);
+  h->tcount = 0;
+  *out = h;
+  return DNS_OK;
+}
+
+extern "C" int dns_destroy(dns_handle *h) {
+  if (!h) return DNS_OK;
+  (void)hipStreamSynchronize(h->stream);
+  destroy_all(h);
+  return DNS_OK;
+}
+
+extern "C" int dns_ygrid(int ny, double dyq, int ppp, double *yu, double *yv, double *dthdyu, double *dthdyv) {
+  if (ny < 6 || ppp < 1 || !yu || !yv || !dthdyu || !dthdyv) return fail(DNS_ERR_ARG, "dns_ygrid: bad argument");
+  dns_handle h;
+  h.prm.ny = ny;
+  h.prm.dyq = dyq;
+  h.prm.ppp = ppp;
+  h.prm.alp = h.prm.bet = h.prm.Re = 1.0;
+  h.ny = ny;
+  h.nyv = ny - 2;
+  h.nyu = ny - 1;
+  h.Nx = h.Nz = 4;
+  host_grid(&h);
+  memcpy(yu, h.yu.data(), h.yu.size() * 8);
+  memcpy(yv, h.yv.data(), h.yv.size() * 8);
+  memcpy(dthdyu, h.dthdyu.data(), h.dthdyu.size() * 8);
+  memcpy(dthdyv, h.dthdyv.data(), h.dthdyv.size() * 8);
+  return DNS_OK;
+}
+
+extern "C" int dns_sizes(const dns_handle *h, long long *out) {
+  if (!h || !out) return fail(DNS_ERR_ARG, "dns_sizes: null argument");
+  const long long v[10] = {h->Nx, h->Nz, h->Kx, h->C, h->nyu, h->nyv, h->Mx, h->Mz, h->ny,
+                           (long long)(h->bytes >> 20)};
+  memcpy(out, v, sizeof v);
+  return DNS_OK;
+}
+
+extern "C" int dns_setup_arrays(const dns_handle *h, double *yu, double *yv, double *dthdyu, double *dthdyv,
+                                double *dyu2i, double *dyv2i, double *w) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_setup_arrays: null handle");
+  if (yu) memcpy(yu, h->yu.data(), h->yu.size() * 8);
+  if (yv) memcpy(yv, h->yv.data(), h->yv.size() * 8);
+  if (dthdyu) memcpy(dthdyu, h->dthdyu.data(), h->dthdyu.size() * 8);
+  if (dthdyv) memcpy(dthdyv, h->dthdyv.data(), h->dthdyv.size() * 8);
+  if (dyu2i) memcpy(dyu2i, h->dyu2i.data(), h->dyu2i.size() * 8);
+  if (dyv2i) memcpy(dyv2i, h->dyv2i.data(), h->dyv2i.size() * 8);
+  if (w) {
+    w[0] = h->dtv; w[1] = h->gw1; w[2] = h->gw2; w[3] = h->gbc; w[4] = h->gi1; w[5] = h->gi2;
+  }
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+static ColArgs col_args(dns_handle *h, int kRK) {
+  ColArgs a;
+  memset(&a, 0, sizeof a);
+  a.Kx = h->Kx; a.Nx = h->Nx; a.Nz = h->Nz; a.C = h->C; a.nyu = h->nyu; a.nyv = h->nyv;
+  a.kRK = kRK; a.flag_ctpress = h->prm.flag_ctpress;
+  a.fixed = h->prm.fixed_dt > 0.0; a.fixed_dt = h->prm.fixed_dt;
+  a.Re = h->prm.Re; a.alp = h->prm.alp; a.bet = h->prm.bet; a.CFL = h->prm.CFL; a.dtv = h->dtv;
+  const int q = kRK - 1;
+  a.aRK = h->aRK[q]; a.bRK = h->bRK[q]; a.gRK = h->gRK[q]; a.cRK = h->cRK[q]; a.dRK = h->dRK[q];
+  a.gw1 = h->gw1; a.gw2 = h->gw2; a.gbc = h->gbc; a.gi1 = h->gi1; a.gi2 = h->gi2;
+  a.dyp = h->yu[2] - h->yu[1];
+  a.yu = h->d_yu; a.yv = h->d_yv; a.dthdyu = h->d_dthdyu; a.dthdyv = h->d_dthdyv;
+  a.dyu2i = h->d_dyu2i; a.dyv2i = h->d_dyv2i;
+  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi; a.div = h->div;
+  a.du1 = h->du1; a.du2 = h->du2; a.du3 = h->du3; a.Nu1 = h->Nu1; a.Nu2 = h->Nu2; a.Nu3 = h->Nu3;
+  a.Nxz1 = h->Nxz1; a.Nxz2 = h->Nxz2; a.Nxz3 = h->Nxz3; a.uvf = h->uvf; a.vvc = h->vvc; a.wvf = h->wvf;
+  a.w3u = h->w3u; a.w3v = h->w3v; a.cP = h->cP; a.mscr = h->mscr;
+  a.st = h->st;
+  return a;
+}
+
+static PlaneArgs plane_args(dns_handle *h, int kRK) {
+  PlaneArgs a;
+  memset(&a, 0, sizeof a);
+  a.Mx = h->Mx; a.Mz = h->Mz; a.Nx = h->Nx; a.Nz = h->Nz; a.Kx = h->Kx; a.C = h->C;
+  a.nyu = h->nyu; a.nyv = h->nyv; a.kRK = kRK;
+  a.alp = h->prm.alp; a.bet = h->prm.bet;
+  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.yu = h->d_yu; a.yv = h->d_yv;
+  a.I = h->I; a.P = h->P; a.twz = h->twz; a.twx = h->twx; a.twh = h->twh;
+  a.Nxz1 = h->Nxz1; a.Nxz2 = h->Nxz2; a.Nxz3 = h->Nxz3; a.uvf = h->uvf; a.vvc = h->vvc; a.wvf = h->wvf;
+  a.cfl = &h->st->cfl[0];
+  return a;
+}
+
+static int check_kRK(dns_handle *h, int kRK, const char *who) {
+  if (!h) return fail(DNS_ERR_ARG, "%s: null handle", who);
+  if (kRK < 1 || kRK > 3) return fail(DNS_ERR_ARG, "%s: kRK=%d not in 1..3", who, kRK);
+  if (!h->uploaded) return fail(DNS_ERR_STATE, "%s: no fields uploaded", who);
+  return DNS_OK;
+}
+
+static void tmark(dns_handle *h, int q) {
+  if (h->timing) (void)hipEventRecord(h->ev[q], h->stream);
+}
+
+// nonlinear plane stage + CFL maxima (FOU3D.f90:98-303 up to the column part)
+static int plane_stage(dns_handle *h, int kRK) {
+  if (kRK == 1) {
+    dnsk::launch_zero(&h->st->cfl[0], 3, h->stream);  // cfl[0], cfl[1], divmax
+    dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, &h->st->cfl[0], h->stream);
+  }
+  PlaneArgs a = plane_args(h, kRK);
+  const int total = 2 * h->nyu;
+  const bool one_group = h->batch >= total;  // per-kernel event marks only with one launch per kernel
+  tmark(h, 0);
+  for (int w0 = 0; w0 < total; w0 += h->batch) {
+    const int nw = std::min(h->batch, total - w0);
+    a.w0 = w0;
+    for (int which = 0; which < 3; which++) {
+      if (dnsk::launch_plane_stage_split(a, nw, h->fp32, h->stream, which))
+        return fail(DNS_ERR_ARG, "plane stage: sizes %d x %d not instantiated", h->Mx, h->Mz);
+      if (one_group) tmark(h, which + 1);
+    }
+  }
+  if (!one_group) {
+    tmark(h, 1);
+    tmark(h, 2);
+    tmark(h, 3);
+  }
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+extern "C" int dns_upload_fields(dns_handle *h, const double *u1, const double *u2, const double *u3,
+                                 const double *p, double t, int iter, double QxT) {
+  if (!h || !u1 || !u2 || !u3 || !p) return fail(DNS_ERR_ARG, "dns_upload_fields: null argument");
+  const size_t C = h->C;
+  struct F { const double *src; double2 *dst; int rows; } f[4] = {
+      {u1, h->u1, h->nyu + 2}, {u2, h->u2, h->nyv + 2}, {u3, h->u3, h->nyu + 2}, {p, h->p, h->nyu}};
+  for (auto &x : f) {
+    HIPCHK(hipMemcpyAsync(h->stage, x.src, C * x.rows * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    dnsk::launch_ref_to_dev(h->stage, x.dst, (int)C, x.rows, h->stream);
+  }
+  // getini: Qx of mode (0,0) (flowrateIm, littleharsh.f90:473-492) and QxT (start.f90:1255-1263)
+  double Qx = 0.0;
+  for (int j = 1; j <= h->nyv + 1; j++) Qx = Qx + (h->yv[j] - h->yv[j - 1]) * u1[2 * j];
+  DnsState s0;
+  memset(&s0, 0, sizeof s0);
+  s0.t = t;
+  s0.iter = iter;
+  s0.mpgx = h->prm.mpgx;
+  s0.Qx = Qx;
+  s0.QxT = QxT > 0.0 ? QxT : Qx;
+  HIPCHK(hipMemcpyAsync(h->st, &s0, sizeof s0, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemsetAsync(h->Nu1, 0, C * h->nyu * sizeof(double2), h->stream));
+  HIPCHK(hipMemsetAsync(h->Nu2, 0, C * h->nyu * sizeof(double2), h->stream));
+  HIPCHK(hipMemsetAsync(h->Nu3, 0, C * h->nyu * sizeof(double2), h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->uploaded = true;
+  return DNS_OK;
+}
+
+static int download_dev(dns_handle *h, const double2 *dev, double *dst, int rows) {
+  const size_t C = h->C;
+  dnsk::launch_dev_to_ref(dev, h->stage, (int)C, rows, h->stream);
+  HIPCHK(hipMemcpyAsync(dst, h->stage, C * rows * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DNS_OK;
+}
+
+static int upload_dev(dns_handle *h, double2 *dev, const double *src, int rows) {
+  const size_t C = h->C;
+  HIPCHK(hipMemcpyAsync(h->stage, src, C * rows * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  dnsk::launch_ref_to_dev(h->stage, dev, (int)C, rows, h->stream);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DNS_OK;
+}
+
+extern "C" int dns_download_fields(dns_handle *h, double *u1, double *u2, double *u3, double *p) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_fields: null handle");
+  int rc = 0;
+  if (u1 && (rc = download_dev(h, h->u1, u1, h->nyu + 2))) return rc;
+  if (u2 && (rc = download_dev(h, h->u2, u2, h->nyv + 2))) return rc;
+  if (u3 && (rc = download_dev(h, h->u3, u3, h->nyu + 2))) return rc;
+  if (p && (rc = download_dev(h, h->p, p, h->nyu))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_download_nonlinear(dns_handle *h, double *Nu1, double *Nu2, double *Nu3) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_nonlinear: null handle");
+  int rc = 0;
+  if (Nu1 && (rc = download_dev(h, h->Nu1, Nu1, h->nyu))) return rc;
+  if (Nu2 && (rc = download_dev(h, h->Nu2, Nu2, h->nyv))) return rc;
+  if (Nu3 && (rc = download_dev(h, h->Nu3, Nu3, h->nyu))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_upload_nonlinear(dns_handle *h, const double *Nu1, const double *Nu2, const double *Nu3) {
+  if (!h || !Nu1 || !Nu2 || !Nu3) return fail(DNS_ERR_ARG, "dns_upload_nonlinear: null argument");
+  int rc = 0;
+  if ((rc = upload_dev(h, h->Nu1, Nu1, h->nyu))) return rc;
+  if ((rc = upload_dev(h, h->Nu2, Nu2, h->nyv))) return rc;
+  if ((rc = upload_dev(h, h->Nu3, Nu3, h->nyu))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_download_rhs(dns_handle *h, double *du1, double *du2, double *du3) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_rhs: null handle");
+  const size_t C = h->C;
+  int rc = 0;  // interior rows 1..nyu / 1..nyv start one row into the device arrays
+  if (du1 && (rc = download_dev(h, h->du1 + C, du1, h->nyu))) return rc;
+  if (du2 && (rc = download_dev(h, h->du2 + C, du2, h->nyv))) return rc;
+  if (du3 && (rc = download_dev(h, h->du3 + C, du3, h->nyu))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_download_flux(dns_handle *h, double *uv_f, double *vv_c, double *wv_f) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_flux: null handle");
+  int rc = 0;
+  if (uv_f && (rc = download_dev(h, h->uvf, uv_f, h->nyv + 2))) return rc;
+  if (vv_c && (rc = download_dev(h, h->vvc, vv_c, h->nyu + 2))) return rc;
+  if (wv_f && (rc = download_dev(h, h->wvf, wv_f, h->nyv + 2))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_download_pressure_correction(dns_handle *h, double *psi, double *div) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_pressure_correction: null handle");
+  int rc = 0;
+  if (psi && (rc = download_dev(h, h->psi, psi, h->nyu))) return rc;
+  if (div && (rc = download_dev(h, h->div, div, h->nyu))) return rc;
+  return DNS_OK;
+}
+
+extern "C" int dns_get_scalars(dns_handle *h, double *s) {
+  if (!h || !s) return fail(DNS_ERR_ARG, "dns_get_scalars: null argument");
+  DnsState x;
+  HIPCHK(hipMemcpyAsync(&x, h->st, sizeof x, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double v[10] = {x.t, x.dt, x.dti, x.mpgx, x.Qx, x.dgx, x.dtc, x.QxT, x.iter, x.divmax};
+  memcpy(s, v, sizeof v);
+  if (x.nonfinite != 0.0) return fail(DNS_ERR_NONFINITE, "non-finite velocity reached the CFL maxima");
+  return DNS_OK;
+}
+
+extern "C" int dns_set_dt(dns_handle *h, double dt) {
+  if (!h || !(dt > 0)) return fail(DNS_ERR_ARG, "dns_set_dt: bad argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double v[3];
+  HIPCHK(hipMemcpy(v, h->st, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  v[1] = dt;
+  v[2] = 1.0 / dt;
+  HIPCHK(hipMemcpy(h->st, v, 3 * sizeof(double), hipMemcpyHostToDevice));
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// staged entry points
+extern "C" int dns_rhs0(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_rhs0");
+  if (rc) return rc;
+  dnsk::launch_rhs0(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_nonlinear(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_nonlinear");
+  if (rc) return rc;
+  if ((rc = plane_stage(h, kRK))) return rc;
+  dnsk::launch_nl_col(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_solveU(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_solveU");
+  if (rc) return rc;
+  dnsk::launch_solveU(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_meanflow_ctU(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_meanflow_ctU");
+  if (rc) return rc;
+  dnsk::launch_meanflow(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_solveP(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_solveP");
+  if (rc) return rc;
+  dnsk::launch_solveP(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_vcorr(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_vcorr");
+  if (rc) return rc;
+  dnsk::launch_zero(&h->st->divmax, 1, h->stream);
+  dnsk::launch_vcorr(col_args(h, kRK), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_meanflow_ctP(dns_handle *h) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_meanflow_ctP: null handle");
+  dnsk::launch_qx(col_args(h, 1), h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// fused path
+static int fused_substep(dns_handle *h, int kRK) {
+  int rc = plane_stage(h, kRK);
+  if (rc) return rc;
+  ColArgs a = col_args(h, kRK);
+  dnsk::launch_fused_up(a, h->stream);
+  tmark(h, 4);
+  dnsk::launch_fused_down(a, h->stream);
+  tmark(h, 5);
+  if (h->prm.flag_ctpress == 0) dnsk::launch_meanflow(a, h->stream);
+  dnsk::launch_fused_final(a, h->stream);
+  tmark(h, 6);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_rk3_substep(dns_handle *h, int kRK) {
+  int rc = check_kRK(h, kRK, "dns_rk3_substep");
+  if (rc) return rc;
+  return fused_substep(h, kRK);
+}
+
+static void accumulate_timing(dns_handle *h) {
+  float ms[7];
+  (void)hipEventSynchronize(h->ev[6]);
+  for (int q = 0; q < 6; q++) (void)hipEventElapsedTime(&ms[q], h->ev[q], h->ev[q + 1]);
+  (void)hipEventElapsedTime(&ms[6], h->ev[0], h->ev[6]);
+  for (int q = 0; q < 7; q++) h->tsum[q] += ms[q];
+  h->tcount++;
+}
+
+extern "C" int dns_rk3_step(dns_handle *h, int nsteps) {
+  int rc = check_kRK(h, 1, "dns_rk3_step");
+  if (rc) return rc;
+  for (int s = 0; s < nsteps; s++) {
+    for (int kRK = 1; kRK <= 3; kRK++) {
+      if ((rc = fused_substep(h, kRK))) return rc;
+      if (h->timing) accumulate_timing(h);
+    }
+  }
+  return DNS_OK;
+}
+
+extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
+  int rc = check_kRK(h, 1, "dns_rk3_step_graph");
+  if (rc) return rc;
+  if (!h->graph) {
+    const bool tm = h->timing;
+    h->timing = false;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int kRK = 1; kRK <= 3 && rc == 0; kRK++) rc = fused_substep(h, kRK);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->timing = tm;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(DNS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(DNS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(h->graph, h->stream));
+  return DNS_OK;
+}
+
+extern "C" int dns_sync(dns_handle *h) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_sync: null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DNS_OK;
+}
+
+extern "C" int dns_set_timing(dns_handle *h, int on) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_set_timing: null handle");
+  h->timing = on != 0;
+  memset(h->tsum, 0, sizeof h->tsum);
+  h->tcount = 0;
+  return DNS_OK;
+}
+
+extern "C" int dns_get_timing(dns_handle *h, double *t, int *n) {
+  if (!h || !t) return fail(DNS_ERR_ARG, "dns_get_timing: null argument");
+  for (int q = 0; q < 7; q++) t[q] = h->tcount ? h->tsum[q] / h->tcount : 0.0;
+  if (n) *n = h->tcount;
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+extern "C" int dns_plane_fft_roundtrip(dns_handle *h, const double *d_in, double *d_phys, double *d_back,
+                                       int nplanes) {
+  if (!h || !d_in || !d_phys || !d_back || nplanes < 0) return fail(DNS_ERR_ARG, "dns_plane_fft_roundtrip: bad argument");
+  if (nplanes == 0) return DNS_OK;
+  if (dnsk::launch_plane_roundtrip(h->Mx, h->Mz, d_in, d_phys, d_back, nplanes, h->twx64, h->twh64, h->twz64,
+                                   h->stream))
+    return fail(DNS_ERR_ARG, "dns_plane_fft_roundtrip: %d x %d not instantiated", h->Mx, h->Mz);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_modes_to_planes(dns_handle *h, const double *d_cols, int nrows, double *d_planes) {
+  if (!h || !d_cols || !d_planes || nrows <= 0) return fail(DNS_ERR_ARG, "dns_modes_to_planes: bad argument");
+  dnsk::launch_modes_to_planes(reinterpret_cast<const double2 *>(d_cols), d_planes, h->C, nrows, h->Kx, h->Nz,
+                               h->Mx, h->Mz, h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+extern "C" int dns_planes_to_modes(dns_handle *h, const double *d_planes, int nrows, double *d_cols) {
+  if (!h || !d_cols || !d_planes || nrows <= 0) return fail(DNS_ERR_ARG, "dns_planes_to_modes: bad argument");
+  dnsk::launch_planes_to_modes(d_planes, reinterpret_cast<double2 *>(d_cols), h->C, nrows, h->Kx, h->Nz, h->Mx,
+                               h->Mz, h->stream);
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}

@@ -145,37 +145,31 @@ def load_state(prefix_dir: str, tag: str, ny: int):
 
 
 # ---------------------------------------------------------------------------------------------
-# input.in : positional text file read with fixed formats (start.f90:86-252, FORMATs :485-488)
+# input.in : positional text file, one value per line at fixed columns, read with the FORMATs
+# (7X,D10.1), (7X,I10), (12X,I5), (8X,A50) (start.f90:86-252, 485-488).  Line numbers (1-based):
+_LINES = {"Re": (10, "r"), "alp": (13, "r"), "bet": (14, "r"), "Mx": (15, "i"), "Mz": (17, "i"),
+          "ny": (19, "i"), "dyq": (20, "r"), "ppp": (21, "i"), "flag_init": (24, "i"), "fnmeb": (28, "a"),
+          "dirin": (30, "a"), "dirout": (31, "a"), "CFL": (34, "r"), "nwrite": (35, "i"),
+          "flag_ctpress": (36, "i5"), "mpgx": (38, "r"), "maxt": (44, "r")}
 
-_INPUT_KEYS = ["Re", "alp", "bet", "Ngalx1", "Ngalx2", "Ngalz1", "Ngalz2", "ny", "dyq", "pp",
-               "flgin", "fnmeb", "dirin", "dirout", "CFL", "nwrite", "flpress", "mpgx", "maxerr",
-               "maxt", "maxA", "type"]
+
+def _field(line: str, kind: str):
+    if kind == "a":
+        return line[8:58].strip()
+    txt = line[12:17] if kind == "i5" else line[7:17]
+    txt = txt.strip()
+    if kind in ("i", "i5"):
+        return int(txt) if txt else 0
+    return float(txt.replace("d", "e").replace("D", "e")) if txt else 0.0
 
 
 def read_input(path: str) -> dict:
-    """Parse the reference's ``input.in`` by key (the reference reads it by line position; the keys
-    of the shipped files are unique, so reading by key gives the same values)."""
-    vals = {}
+    """Parse an input.in the way the reference's set-up does: by line position and column."""
     with open(path) as f:
-        for line in f:
-            m = re.match(r"^([A-Za-z][A-Za-z0-9_]*)\s*=\s*(\S+)", line)
-            if not m:
-                continue
-            key, v = m.group(1), m.group(2)
-            if key in ("fnmeb", "dirin", "dirout", "dirlist", "heading"):
-                vals[key] = v
-                continue
-            try:
-                vals[key] = float(v.replace("d", "e").replace("D", "e"))
-            except ValueError:
-                vals[key] = v
-    cfg = {
-        "Re": vals["Re"], "alp": vals["alp"], "bet": vals["bet"],
-        "Mx": int(vals["Ngalx1"]), "Mz": int(vals["Ngalz1"]), "ny": int(vals["ny"]),
-        "dyq": vals["dyq"], "ppp": int(vals["pp"]), "CFL": vals["CFL"],
-        "flag_ctpress": int(vals.get("flpress", 1)), "mpgx": vals.get("mpgx", 0.0),
-        "flag_init": int(vals.get("flgin", 1)), "fnmeb": vals.get("fnmeb", ""),
-        "dirin": vals.get("dirin", "./input/"), "dirout": vals.get("dirout", "./output/"),
-        "nwrite": int(vals.get("nwrite", 1)), "maxt": vals.get("maxt", 1e30),
-    }
+        lines = f.read().split("\n")
+    cfg = {k: _field(lines[n - 1], kind) for k, (n, kind) in _LINES.items()}
+    if cfg["flag_ctpress"] not in (0, 1):
+        raise ValueError("Wrong presure flag")  # start.f90:203-204
+    if cfg["flag_ctpress"] == 0:
+        cfg["mpgx"] = 0.0
     return cfg

@@ -1,0 +1,56 @@
+"""Algorithmic (compulsory) HBM bytes of each kernel of one RK3 substep, and of the whole step.
+
+Every kernel is charged the bytes it must read and write at least once: each input row read once, each
+output row written once, no re-reads, no cache effects.  Spectral rows are complex FP64 (16 B per
+retained mode, ncol = (Nx/2+1)*Nz modes per row).  Plane-stage intermediates hold Kx x Mz complex values
+per field in the plane precision (16 B FP64, 8 B mixed).
+
+Rows per field (device layout):
+  u1, u3: 0..nyu+1     u2: 0..nyv+1     p, psi, Nu1, Nu3, Nxz1, Nxz3: 1..nyu    Nu2, Nxz2: 1..nyv
+  uv_f, wv_f: 0..nyv+1 vv_c: 0..nyu+1   w3u: nyu+2 reals, w3v: nyv+2 reals, cP: nyu reals
+"""
+from __future__ import annotations
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def kernel_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    Kx = Nx // 2 + 1
+    C = Kx * Nz
+    nyv = ny - 2
+    nyu = nyv + 1
+    S = 16 * C                 # one spectral row
+    R = 8 * C                  # one row of real per-column scratch
+    I = Kx * Mz * (8 if fp32 else 16)   # one plane-stage field
+    npl = 2 * nyu              # u-planes 1..nyu, v-planes 1..nyv+1
+    b = {}
+    # z-inverse: u1, u3 rows 1..nyu+1, u2 rows 0..nyv+1 read once; 3 fields per plane written
+    b["zinv"] = 3 * (nyu + 1) * S + 3 * npl * I
+    # x-pass: 3 fields per plane in; 4 products per u-plane, 2 per v-plane out
+    b["xpass"] = 3 * npl * I + (4 * nyu + 2 * nyu) * I
+    # z-forward: products in; Nxz1, Nxz3, vv_c per u-plane; Nxz2 (j <= nyv), uv_f, wv_f per v-plane
+    b["zfwd"] = (4 * nyu + 2 * nyu) * I + (3 * nyu + 2 * nyu + nyv) * S
+    # column sweep up: reads u1,u3 (nyu+2), u2 (nyv+2), p, Nu1/3 old, Nxz1/3, vv_c (nyu each), Nu2 old, Nxz2
+    # (nyv each), uv_f, wv_f (nyu+1 each); writes u1,u3,u2 eliminated, Nu1/2/3 new, w3u, w3v
+    rd = (2 * (nyu + 2) + (nyv + 2) + 6 * nyu + 2 * nyv + 2 * (nyu + 1)) * S
+    wr = (2 * (nyu + 2) + (nyv + 2) + 2 * nyu + nyv) * S + ((nyu + 2) + (nyv + 2)) * R
+    b["col_up"] = rd + wr
+    # column sweep down: reads u1,u3,u2 (eliminated) and w3u, w3v; writes u1,u3 (0..nyu), u2 (0..nyv),
+    # psi (1..nyu), cP
+    b["col_down"] = ((2 * (nyu + 2) + (nyv + 2)) * S + ((nyu + 2) + (nyv + 2)) * R
+                     + (2 * (nyu + 1) + (nyv + 1) + nyu) * S + nyu * R)
+    # final sweep: reads psi, cP, p (1..nyu), u1, u3 (1..nyu), u2 (1..nyv); writes p, u1, u3, u2 (all rows)
+    b["col_final"] = ((nyu + nyu + 2 * nyu + nyv) * S + nyu * R
+                      + (nyu + 2 * (nyu + 2) + (nyv + 2)) * S)
+    b["substep"] = sum(b[k] for k in ("zinv", "xpass", "zfwd", "col_up", "col_down", "col_final"))
+    b["step"] = 3 * b["substep"]
+    return b
+
+
+def roofline(kernel: str, Mx: int, Mz: int, ny: int, fp32: bool, avg_ms: float) -> dict:
+    """Achieved bandwidth of one kernel from its algorithmic bytes and its measured average duration."""
+    by = kernel_bytes(Mx, Mz, ny, fp32)[kernel]
+    gbs = by / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "bytes_per_launch": by}

@@ -1,0 +1,220 @@
+"""ctypes binding of the C-ABI in include/dns_hip.h (libdns_hip.so, built next to this file).
+
+This is plumbing for Python callers (tests, bench): the product path is the HIP library; there is no
+Python or CPU fallback.  Loading fails loudly when the library is missing.
+
+Column fields cross the boundary in the reference layout x(j, column): numpy complex128 arrays of
+shape [ncol, nrows] (C-contiguous, so j is fastest, as in Fortran x(j, column)).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libdns_hip.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class DnsError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("Mx", ctypes.c_int), ("Mz", ctypes.c_int), ("ny", ctypes.c_int), ("ppp", ctypes.c_int),
+                ("flag_ctpress", ctypes.c_int), ("precision", ctypes.c_int),
+                ("Re", ctypes.c_double), ("alp", ctypes.c_double), ("bet", ctypes.c_double),
+                ("CFL", ctypes.c_double), ("mpgx", ctypes.c_double), ("dyq", ctypes.c_double),
+                ("fixed_dt", ctypes.c_double), ("plane_batch", ctypes.c_int),
+                ("reserved", ctypes.c_int * 7)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_sizes", "dns_setup_arrays",
+           "dns_upload_fields", "dns_download_fields", "dns_download_nonlinear", "dns_upload_nonlinear",
+           "dns_download_rhs", "dns_download_flux", "dns_download_pressure_correction", "dns_get_scalars",
+           "dns_set_dt", "dns_rhs0", "dns_nonlinear", "dns_solveU", "dns_meanflow_ctU", "dns_solveP",
+           "dns_vcorr", "dns_meanflow_ctP", "dns_rk3_substep", "dns_rk3_step", "dns_rk3_step_graph",
+           "dns_plane_fft_roundtrip", "dns_modes_to_planes", "dns_planes_to_modes", "dns_sync",
+           "dns_set_timing", "dns_get_timing"]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise DnsError("%s is missing: run __graft_entry__.build() (no CPU fallback exists)" % LIBPATH)
+    L = ctypes.CDLL(LIBPATH)
+    L.dns_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    L.dns_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.dns_destroy.argtypes = [vp]
+    L.dns_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.dns_setup_arrays.argtypes = [vp] + [_dp] * 7
+    L.dns_ygrid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int] + [_dp] * 4
+    L.dns_upload_fields.argtypes = [vp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    for n in ("dns_download_fields",):
+        getattr(L, n).argtypes = [vp, _dp, _dp, _dp, _dp]
+    for n in ("dns_download_nonlinear", "dns_upload_nonlinear", "dns_download_rhs", "dns_download_flux"):
+        getattr(L, n).argtypes = [vp, _dp, _dp, _dp]
+    L.dns_download_pressure_correction.argtypes = [vp, _dp, _dp]
+    L.dns_get_scalars.argtypes = [vp, _dp]
+    L.dns_set_dt.argtypes = [vp, ctypes.c_double]
+    for n in ("dns_rhs0", "dns_nonlinear", "dns_solveU", "dns_meanflow_ctU", "dns_solveP", "dns_vcorr",
+              "dns_rk3_substep", "dns_rk3_step", "dns_rk3_step_graph", "dns_set_timing"):
+        getattr(L, n).argtypes = [vp, ctypes.c_int]
+    L.dns_meanflow_ctP.argtypes = [vp]
+    L.dns_sync.argtypes = [vp]
+    L.dns_plane_fft_roundtrip.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+    L.dns_modes_to_planes.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.dns_planes_to_modes.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.dns_get_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
+    _lib = L
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise DnsError("status %d: %s" % (rc, lib().dns_last_error().decode()))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def ygrid(ny, dyq=10.0, ppp=5):
+    """Host-only y grid (start.f90:665-806): dict(yu, yv, dthdyu, dthdyv)."""
+    yu, yv, du, dv = np.zeros(ny + 1), np.zeros(ny), np.zeros(ny + 1), np.zeros(ny)
+    _chk(lib().dns_ygrid(int(ny), float(dyq), int(ppp), _p(yu), _p(yv), _p(du), _p(dv)))
+    return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv)
+
+
+class DNS:
+    """One channel on the GPU (one dns_handle)."""
+
+    def __init__(self, Mx, Mz, ny, Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
+                 flag_ctpress=1, precision=PREC_FP64, fixed_dt=0.0, plane_batch=0):
+        L = lib()
+        self.prm = Params(Mx=Mx, Mz=Mz, ny=ny, ppp=ppp, flag_ctpress=flag_ctpress, precision=precision,
+                          Re=Re, alp=alp, bet=bet, CFL=CFL, mpgx=mpgx, dyq=dyq, fixed_dt=fixed_dt,
+                          plane_batch=plane_batch)
+        h = ctypes.c_void_p()
+        _chk(L.dns_create(ctypes.byref(self.prm), ctypes.byref(h)))
+        self.h = h
+        s = (ctypes.c_longlong * 10)()
+        _chk(L.dns_sizes(self.h, s))
+        self.Nx, self.Nz, self.Kx, self.C, self.nyu, self.nyv = (int(s[q]) for q in range(6))
+        self.Mx, self.Mz, self.ny, self.mib = int(s[6]), int(s[7]), int(s[8]), int(s[9])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- shapes in the reference layout
+    def shape(self, name):
+        C, ny = self.C, self.ny
+        return {"u1": (C, ny + 1), "u3": (C, ny + 1), "u2": (C, ny), "p": (C, ny - 1),
+                "Nu1": (C, ny - 1), "Nu3": (C, ny - 1), "Nu2": (C, ny - 2)}[name]
+
+    def setup_arrays(self):
+        ny = self.ny
+        yu, yv, du, dv = np.zeros(ny + 1), np.zeros(ny), np.zeros(ny + 1), np.zeros(ny)
+        d2u, d2v, w = np.zeros((ny + 1, 3)), np.zeros((ny, 3)), np.zeros(6)
+        _chk(lib().dns_setup_arrays(self.h, *[_p(a) for a in (yu, yv, du, dv, d2u, d2v, w)]))
+        return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv, dyu2i=d2u, dyv2i=d2v, dtv=w[0], gw=w[1:3], gbc=w[3],
+                    gi=w[4:6])
+
+    def upload(self, u1, u2, u3, p, t=0.0, it=0, QxT=0.0):
+        arrs = []
+        for name, a in (("u1", u1), ("u2", u2), ("u3", u3), ("p", p)):
+            a = np.ascontiguousarray(a, dtype=np.complex128)
+            if a.shape != self.shape(name):
+                raise DnsError("%s has shape %s, expected %s" % (name, a.shape, self.shape(name)))
+            arrs.append(a)
+        _chk(lib().dns_upload_fields(self.h, *[_p(a.view(np.float64)) for a in arrs], float(t), int(it),
+                                     float(QxT)))
+
+    def download(self):
+        out = [np.zeros(self.shape(n), np.complex128) for n in ("u1", "u2", "u3", "p")]
+        _chk(lib().dns_download_fields(self.h, *[_p(a.view(np.float64)) for a in out]))
+        return out
+
+    def download_nonlinear(self):
+        out = [np.zeros(self.shape(n), np.complex128) for n in ("Nu1", "Nu2", "Nu3")]
+        _chk(lib().dns_download_nonlinear(self.h, *[_p(a.view(np.float64)) for a in out]))
+        return out
+
+    def upload_nonlinear(self, Nu1, Nu2, Nu3):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (Nu1, Nu2, Nu3)]
+        _chk(lib().dns_upload_nonlinear(self.h, *[_p(a.view(np.float64)) for a in arrs]))
+
+    def download_rhs(self):
+        out = [np.zeros(self.shape(n), np.complex128) for n in ("Nu1", "Nu2", "Nu3")]
+        _chk(lib().dns_download_rhs(self.h, *[_p(a.view(np.float64)) for a in out]))
+        return out
+
+    def download_flux(self):
+        out = [np.zeros(self.shape(n), np.complex128) for n in ("u2", "u1", "u2")]
+        _chk(lib().dns_download_flux(self.h, *[_p(a.view(np.float64)) for a in out]))
+        return out
+
+    def download_psi_div(self):
+        out = [np.zeros(self.shape("p"), np.complex128) for _ in range(2)]
+        _chk(lib().dns_download_pressure_correction(self.h, *[_p(a.view(np.float64)) for a in out]))
+        return out
+
+    def scalars(self):
+        s = np.zeros(10)
+        _chk(lib().dns_get_scalars(self.h, _p(s)))
+        return dict(zip(["t", "dt", "dti", "mpgx", "Qx", "dgx", "dtc", "QxT", "iter", "divmax"], s))
+
+    def set_dt(self, dt):
+        _chk(lib().dns_set_dt(self.h, float(dt)))
+
+    # ---- staged entry points (one per reference subroutine)
+    def rhs0(self, k): _chk(lib().dns_rhs0(self.h, k))
+    def nonlinear(self, k): _chk(lib().dns_nonlinear(self.h, k))
+    def solveU(self, k): _chk(lib().dns_solveU(self.h, k))
+    def meanflow_ctU(self, k): _chk(lib().dns_meanflow_ctU(self.h, k))
+    def solveP(self, k): _chk(lib().dns_solveP(self.h, k))
+    def vcorr(self, k): _chk(lib().dns_vcorr(self.h, k))
+    def meanflow_ctP(self): _chk(lib().dns_meanflow_ctP(self.h))
+
+    # ---- fused path
+    def substep(self, k): _chk(lib().dns_rk3_substep(self.h, k))
+    def step(self, n=1): _chk(lib().dns_rk3_step(self.h, n))
+    def step_graph(self, n=1): _chk(lib().dns_rk3_step_graph(self.h, n))
+    def sync(self): _chk(lib().dns_sync(self.h))
+
+    def set_timing(self, on): _chk(lib().dns_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        t = np.zeros(7)
+        n = ctypes.c_int()
+        _chk(lib().dns_get_timing(self.h, _p(t), ctypes.byref(n)))
+        return dict(zinv=t[0], xpass=t[1], zfwd=t[2], col_up=t[3], col_down=t[4], col_final=t[5],
+                    substep=t[6], n=n.value)
+
+    # ---- device-pointer utilities (torch tensors provide the device memory)
+    def plane_fft_roundtrip(self, d_in, d_phys, d_back, n):
+        _chk(lib().dns_plane_fft_roundtrip(self.h, d_in, d_phys, d_back, n))
+
+    def modes_to_planes(self, d_cols, nrows, d_planes):
+        _chk(lib().dns_modes_to_planes(self.h, d_cols, nrows, d_planes))
+
+    def planes_to_modes(self, d_planes, nrows, d_cols):
+        _chk(lib().dns_planes_to_modes(self.h, d_planes, nrows, d_cols))

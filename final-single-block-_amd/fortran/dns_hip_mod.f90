@@ -1,0 +1,201 @@
+! ISO_C_BINDING interface to the C-ABI of include/dns_hip.h (libdns_hip.so).
+!
+! The reference calls its hot path as external Fortran subroutines with implicit interfaces from the
+! main loop (littleharsh.f90:163-233).  This module binds every C entry point by name, and
+! dns_hip_check turns a non-zero status into `error stop` with the library's message, which keeps the
+! reference's fatal-stop error behaviour (start.f90:203-204, rft_buff.f:308-310).
+module dns_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: DNS_OK = 0, DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1
+
+  type, bind(C) :: dns_params
+    integer(c_int) :: Mx, Mz, ny, ppp, flag_ctpress, precision
+    real(c_double) :: Re, alp, bet, CFL, mpgx, dyq, fixed_dt
+    integer(c_int) :: plane_batch
+    integer(c_int) :: reserved(7)
+  end type
+
+  interface
+    function dns_last_error() bind(C, name='dns_last_error')
+      import :: c_ptr
+      type(c_ptr) :: dns_last_error
+    end function
+    integer(c_int) function dns_ygrid(ny, dyq, ppp, yu, yv, dthdyu, dthdyv) bind(C, name='dns_ygrid')
+      import :: c_int, c_double
+      integer(c_int), value :: ny, ppp
+      real(c_double), value :: dyq
+      real(c_double) :: yu(*), yv(*), dthdyu(*), dthdyv(*)
+    end function
+    integer(c_int) function dns_create(prm, h) bind(C, name='dns_create')
+      import :: c_int, c_ptr, dns_params
+      type(dns_params), intent(in) :: prm
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function dns_destroy(h) bind(C, name='dns_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dns_sizes(h, out) bind(C, name='dns_sizes')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: h
+      integer(c_long_long) :: out(10)
+    end function
+    integer(c_int) function dns_setup_arrays(h, yu, yv, dthdyu, dthdyv, dyu2i, dyv2i, w) &
+        bind(C, name='dns_setup_arrays')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: yu(*), yv(*), dthdyu(*), dthdyv(*), dyu2i(*), dyv2i(*), w(*)
+    end function
+    integer(c_int) function dns_upload_fields(h, u1, u2, u3, p, t, iter, QxT) bind(C, name='dns_upload_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double), intent(in) :: u1(*), u2(*), u3(*), p(*)
+      real(c_double), value :: t, QxT
+      integer(c_int), value :: iter
+    end function
+    integer(c_int) function dns_download_fields(h, u1, u2, u3, p) bind(C, name='dns_download_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double) :: u1(*), u2(*), u3(*), p(*)
+    end function
+    integer(c_int) function dns_download_nonlinear(h, Nu1, Nu2, Nu3) bind(C, name='dns_download_nonlinear')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double) :: Nu1(*), Nu2(*), Nu3(*)
+    end function
+    integer(c_int) function dns_upload_nonlinear(h, Nu1, Nu2, Nu3) bind(C, name='dns_upload_nonlinear')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double), intent(in) :: Nu1(*), Nu2(*), Nu3(*)
+    end function
+    integer(c_int) function dns_download_rhs(h, du1, du2, du3) bind(C, name='dns_download_rhs')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double) :: du1(*), du2(*), du3(*)
+    end function
+    integer(c_int) function dns_download_flux(h, uv_f, vv_c, wv_f) bind(C, name='dns_download_flux')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double) :: uv_f(*), vv_c(*), wv_f(*)
+    end function
+    integer(c_int) function dns_download_pressure_correction(h, psi, div) &
+        bind(C, name='dns_download_pressure_correction')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      complex(c_double) :: psi(*), div(*)
+    end function
+    integer(c_int) function dns_get_scalars(h, s) bind(C, name='dns_get_scalars')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: s(10)
+    end function
+    integer(c_int) function dns_set_dt(h, dt) bind(C, name='dns_set_dt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function dns_rhs0(h, kRK) bind(C, name='dns_rhs0')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_nonlinear(h, kRK) bind(C, name='dns_nonlinear')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_solveU(h, kRK) bind(C, name='dns_solveU')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_meanflow_ctU(h, kRK) bind(C, name='dns_meanflow_ctU')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_solveP(h, kRK) bind(C, name='dns_solveP')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_vcorr(h, kRK) bind(C, name='dns_vcorr')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_meanflow_ctP(h) bind(C, name='dns_meanflow_ctP')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dns_rk3_substep(h, kRK) bind(C, name='dns_rk3_substep')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kRK
+    end function
+    integer(c_int) function dns_rk3_step(h, nsteps) bind(C, name='dns_rk3_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function dns_rk3_step_graph(h, nsteps) bind(C, name='dns_rk3_step_graph')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function dns_plane_fft_roundtrip(h, d_in, d_phys, d_back, nplanes) &
+        bind(C, name='dns_plane_fft_roundtrip')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, d_in, d_phys, d_back
+      integer(c_int), value :: nplanes
+    end function
+    integer(c_int) function dns_modes_to_planes(h, d_cols, nrows, d_planes) bind(C, name='dns_modes_to_planes')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, d_cols, d_planes
+      integer(c_int), value :: nrows
+    end function
+    integer(c_int) function dns_planes_to_modes(h, d_planes, nrows, d_cols) bind(C, name='dns_planes_to_modes')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, d_planes, d_cols
+      integer(c_int), value :: nrows
+    end function
+    integer(c_int) function dns_sync(h) bind(C, name='dns_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dns_set_timing(h, on) bind(C, name='dns_set_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function dns_get_timing(h, t, n) bind(C, name='dns_get_timing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: t(7)
+      integer(c_int) :: n
+    end function
+    function c_strlen(s) bind(C, name='strlen')
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: c_strlen
+    end function
+  end interface
+
+contains
+
+  ! status check with the reference's fatal-stop semantics
+  subroutine dns_hip_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == DNS_OK) return
+    n = int(c_strlen(dns_last_error()))
+    call c_f_pointer(dns_last_error(), msg, [n])
+    write(0,'(A,A,I0,A,*(A))') trim(where), ': status ', rc, ': ', msg(1:n)
+    error stop 1
+  end subroutine
+
+end module

@@ -1,0 +1,352 @@
+! Drop-in Fortran driver: the reference's main loop (littleharsh.f90:163-233) calling subroutines with the
+! reference's names and argument lists, each of which forwards to the HIP implementation through the
+! ISO_C_BINDING interface (dns_hip_mod.f90).  Fields live in GPU memory between calls; the host arrays in
+! the argument lists keep the reference's shapes and are synchronised only for output.
+!
+!   littleharsh_hip [nsteps] [fused]
+!
+! Reads input.in in the reference's positional format (start.f90:86-252) and the initial field files
+! ``<dirin>/{u1,u2,u3,p}<fnmeb minus its 2-char prefix>`` (start.f90:1545-1997), runs until t >= maxt or
+! nsteps steps, and writes the final fields to ``<dirout>/{u1,u2,u3,p}_NNNNxNNNNxNNNN_tTTTTT.dat`` with the
+! record layout of record_out (FOU3D.f90:2063-2161).
+
+module lh_state
+  use iso_c_binding
+  use dns_hip
+  implicit none
+  type(c_ptr) :: hdns = c_null_ptr
+  integer :: kRK = 1, solveU_calls = 0
+  integer :: Nspec_x, Nspec_z, Ngal_x, Ngal_z, nyu, nyv, ny, ncol, flag_ctpress, flag_init, nwrite, ppp
+  real(8) :: Re, alp, bet, CFL, mpgx, dyq, maxt, t, dt
+  integer :: iter
+  character(len=120) :: fnameimb, dirin, dirout
+end module
+
+! ---- wrappers with the reference's names (littleharsh.f90:682-993) ----
+subroutine RHS0_u1(du1, u1, Nu1, p, myid)
+  use lh_state
+  implicit none
+  integer myid
+  complex(8) du1(*), u1(*), Nu1(*), p(*)
+  solveU_calls = 0
+  call dns_hip_check(dns_rhs0(hdns, int(kRK, c_int)), 'RHS0_u1')   ! RHS0 of u1, u2 and u3
+end subroutine
+
+subroutine RHS0_u2(du2, u2, Nu2, p, myid)
+  implicit none
+  integer myid
+  complex(8) du2(*), u2(*), Nu2(*), p(*)
+  ! computed together with RHS0_u1 on the GPU
+end subroutine
+
+subroutine RHS0_u3(du3, u3, Nu3, p, myid)
+  implicit none
+  integer myid
+  complex(8) du3(*), u3(*), Nu3(*), p(*)
+  ! computed together with RHS0_u1 on the GPU
+end subroutine
+
+subroutine nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, myid, status, ierr)
+  use lh_state
+  implicit none
+  integer myid, status(*), ierr
+  complex(8) Nu1(*), Nu2(*), Nu3(*), u1(*), u2(*), u3(*), du1(*), du2(*), du3(*), p(*), div(*)
+  call dns_hip_check(dns_nonlinear(hdns, int(kRK, c_int)), 'nonlinear')
+  ierr = 0
+end subroutine
+
+subroutine solveU(u, du, grid, a_grid, myid)
+  use lh_state
+  implicit none
+  integer grid, myid
+  complex(8) u(*), du(*)
+  real(8) a_grid(*)
+  ! the three velocity solves of a substep run in one launch at the first call
+  if (solveU_calls == 0) call dns_hip_check(dns_solveU(hdns, int(kRK, c_int)), 'solveU')
+  solveU_calls = solveU_calls + 1
+end subroutine
+
+subroutine meanflow_ctU(u1, myid)
+  use lh_state
+  implicit none
+  integer myid
+  complex(8) u1(*)
+  call dns_hip_check(dns_meanflow_ctU(hdns, int(kRK, c_int)), 'meanflow_ctU')
+end subroutine
+
+subroutine solveP(p, psi, u1, u2, u3, div, myid)
+  use lh_state
+  implicit none
+  integer myid
+  complex(8) p(*), psi(*), u1(*), u2(*), u3(*), div(*)
+  call dns_hip_check(dns_solveP(hdns, int(kRK, c_int)), 'solveP')
+end subroutine
+
+subroutine v_corr(u1, u2, u3, psi, div, myid, status, ierr)
+  use lh_state
+  implicit none
+  integer myid, status(*), ierr
+  complex(8) u1(*), u2(*), u3(*), psi(*), div(*)
+  call dns_hip_check(dns_vcorr(hdns, int(kRK, c_int)), 'v_corr')
+  ierr = 0
+end subroutine
+
+subroutine meanflow_ctP(u1, myid)
+  use lh_state
+  implicit none
+  integer myid
+  complex(8) u1(*)
+  call dns_hip_check(dns_meanflow_ctP(hdns), 'meanflow_ctP')
+end subroutine
+
+! ---------------------------------------------------------------------------------------------------
+program littleharsh_hip
+  use lh_state
+  implicit none
+  complex(8), allocatable :: u1(:,:), u2(:,:), u3(:,:), p(:,:), psi(:,:), div(:,:)
+  complex(8), allocatable :: Nu1(:,:), Nu2(:,:), Nu3(:,:), du1(:,:), du2(:,:), du3(:,:)
+  real(8), allocatable :: a_ugrid(:,:), a_vgrid(:,:)
+  type(dns_params) :: prm
+  integer :: myid = 0, status(8), ierr, nmax, nargs, iter_start
+  logical :: fused
+  character(len=32) :: arg
+  real(8) :: sc(10)
+
+  nmax = huge(nmax)
+  fused = .false.
+  nargs = command_argument_count()
+  if (nargs >= 1) then
+    call get_command_argument(1, arg)
+    read(arg, *) nmax
+  end if
+  if (nargs >= 2) then
+    call get_command_argument(2, arg)
+    fused = trim(arg) == 'fused'
+  end if
+
+  call read_input('input.in')
+  prm%Mx = Ngal_x; prm%Mz = Ngal_z; prm%ny = ny; prm%ppp = ppp
+  prm%flag_ctpress = flag_ctpress; prm%precision = DNS_PREC_FP64
+  prm%Re = Re; prm%alp = alp; prm%bet = bet; prm%CFL = CFL; prm%mpgx = mpgx; prm%dyq = dyq
+  prm%fixed_dt = 0d0; prm%plane_batch = 0; prm%reserved = 0
+  call dns_hip_check(dns_create(prm, hdns), 'start')
+
+  allocate(u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol))
+  allocate(psi(1:nyu, ncol), div(1:nyu, ncol))
+  allocate(du1(0:nyu+1, ncol), du2(0:nyv+1, ncol), du3(0:nyu+1, ncol))
+  allocate(Nu1(1:nyu, ncol), Nu2(1:nyv, ncol), Nu3(1:nyu, ncol))
+  allocate(a_ugrid(3, 0:nyu+1), a_vgrid(3, 0:nyv+1))
+  u1 = 0d0; u2 = 0d0; u3 = 0d0; p = 0d0; psi = 0d0; div = 0d0
+
+  call read_fields(u1, u2, u3, p)
+  if (flag_init == 1) then   ! start.f90:1251-1267: a borrowed field restarts the clock
+    t = 0d0
+    iter = 0
+  end if
+  call dns_hip_check(dns_upload_fields(hdns, u1, u2, u3, p, t, int(iter, c_int), 0d0), 'getini')
+  iter_start = iter
+  write(*,'(A,I0,A,I0,A,I0,A,ES12.5)') ' littleharsh_hip: ', Nspec_x, ' x ', Nspec_z, ' x ', ny, '  t0 = ', t
+
+  if (fused) then
+    do while (t < maxt .and. iter - iter_start < nmax)
+      call dns_hip_check(dns_rk3_step(hdns, 1_c_int), 'rk3_step')
+      iter = iter + 1
+      call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
+      t = sc(1)
+    end do
+  else
+    ! main loop, littleharsh.f90:163-233
+    do while (t < maxt .and. iter - iter_start < nmax)
+      do kRK = 1, 3
+        call RHS0_u1(du1, u1, Nu1, p, myid)
+        call RHS0_u2(du2, u2, Nu2, p, myid)
+        call RHS0_u3(du3, u3, Nu3, p, myid)
+        call nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, myid, status, ierr)
+        call solveU(u1, du1, 2, a_ugrid, myid)
+        call solveU(u2, du2, 1, a_vgrid, myid)
+        call solveU(u3, du3, 2, a_ugrid, myid)
+        if (flag_ctpress == 0) call meanflow_ctU(u1, myid)
+        call solveP(p, psi, u1, u2, u3, div, myid)
+        call v_corr(u1, u2, u3, psi, div, myid, status, ierr)
+      end do
+      if (flag_ctpress == 1) call meanflow_ctP(u1, myid)
+      iter = iter + 1
+      call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
+      t = sc(1)
+    end do
+  end if
+
+  call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
+  t = sc(1); dt = sc(2); mpgx = sc(4)
+  call dns_hip_check(dns_download_fields(hdns, u1, u2, u3, p), 'finalize')
+  write(*,'(A,I0,A,ES24.16,A,ES24.16,A,ES24.16)') ' iter ', iter, '  t = ', t, '  dt = ', dt, '  Qx = ', sc(5)
+  call write_fields(u1, u2, u3, p)
+  call dns_hip_check(dns_destroy(hdns), 'finalize')
+
+contains
+
+  ! input.in, positional with the reference's FORMATs (start.f90:86-252, 485-488)
+  subroutine read_input(fname)
+    character(*), intent(in) :: fname
+    integer :: i
+10  format(7X, D10.1)
+20  format(7X, I10)
+21  format(12X, I5)
+30  format(8X, A50)
+    open(40, file=fname, form='formatted', status='old')
+    do i = 1, 9
+      read(40, 10)
+    end do
+    read(40, 10) Re
+    read(40, 10)
+    read(40, 10)
+    read(40, 10) alp
+    read(40, 10) bet
+    read(40, 20) Ngal_x
+    read(40, 20)
+    read(40, 20) Ngal_z
+    read(40, 20)
+    read(40, 20) ny
+    read(40, 10) dyq
+    read(40, 20) ppp
+    read(40, 10)
+    read(40, 10)
+    read(40, 20) flag_init
+    read(40, 10)
+    read(40, 10)
+    read(40, 10)
+    read(40, 30) fnameimb
+    read(40, 10)
+    read(40, 30) dirin
+    read(40, 30) dirout
+    read(40, 10)
+    read(40, 10)
+    read(40, 10) CFL
+    read(40, 20) nwrite
+    read(40, 21) flag_ctpress
+    read(40, 10)
+    if (flag_ctpress == 1) then
+      read(40, 10) mpgx
+    else if (flag_ctpress == 0) then
+      read(40, 10)
+      mpgx = 0d0
+    else
+      write(0, *) 'Wrong presure flag'
+      error stop 1
+    end if
+    read(40, 10)
+    read(40, 10)
+    read(40, 10)
+    read(40, 10)
+    read(40, 10)
+    read(40, 10) maxt
+    close(40)
+    Nspec_x = (2*Ngal_x)/3
+    Nspec_z = (2*Ngal_z)/3
+    nyv = ny - 2
+    nyu = nyv + 1
+    ncol = (Nspec_x/2 + 1)*Nspec_z
+  end subroutine
+
+  ! field files (start.f90:1545-1997 reads the same records); planes placed by their stored index j
+  subroutine read_one(name, x, j0, j1, tt, it)
+    character(*), intent(in) :: name
+    integer, intent(in) :: j0, j1
+    complex(8), intent(inout) :: x(j0:j1, ncol)
+    real(8), intent(out) :: tt
+    integer, intent(out) :: it
+    real(8) :: Re2, alp2, bet2, dumm, yj
+    integer :: nband2, dummint(88), jin, idd, nx, nz, ios, k, i, col
+    integer, allocatable :: N2(:,:)
+    real(8), allocatable :: buff(:,:)
+    character(len=240) :: path
+    path = trim(dirin)//'/'//name//trim(fnameimb(3:))
+    open(10, file=path, form='unformatted', status='old')
+    read(10) tt, Re2, alp2, bet2, dumm, nband2, it, dummint
+    allocate(N2(4, 0:nband2+1))
+    read(10) N2
+    read(10)
+    do
+      read(10, iostat=ios) jin, idd, nx, nz, yj
+      if (ios /= 0) exit
+      backspace(10)
+      allocate(buff(nx, nz))
+      read(10) jin, idd, nx, nz, yj, buff
+      if (jin >= j0 .and. jin <= j1) then
+        do k = 1, min(nz, Nspec_z)
+          do i = 0, min(nx/2, Nspec_x/2+1) - 1
+            col = (k-1)*(Nspec_x/2+1) + i + 1
+            x(jin, col) = dcmplx(buff(2*i+1, k), buff(2*i+2, k))
+          end do
+        end do
+      end if
+      deallocate(buff)
+    end do
+    close(10)
+  end subroutine
+
+  subroutine read_fields(u1, u2, u3, p)
+    complex(8), intent(inout) :: u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol)
+    real(8) :: tt
+    integer :: it
+    call read_one('u1', u1, 0, nyu+1, tt, it)
+    t = tt
+    iter = it
+    call read_one('u2', u2, 0, nyv+1, tt, it)
+    call read_one('u3', u3, 0, nyu+1, tt, it)
+    call read_one('p', p, 1, nyu, tt, it)
+  end subroutine
+
+  ! record_out's file layout (FOU3D.f90:2063-2161) for one field
+  subroutine write_one(name, x, j0, j1, y, dthdy, gridid)
+    character(*), intent(in) :: name
+    integer, intent(in) :: j0, j1, gridid
+    complex(8), intent(in) :: x(j0:j1, ncol)
+    real(8), intent(in) :: y(0:), dthdy(0:)
+    integer :: N(4, 0:4), dummint(88), j, k, i, col
+    real(8), allocatable :: buff(:,:)
+    character(len=240) :: path
+    character(len=4) :: e1, e2, e3
+    character(len=5) :: e4
+    write(e1, '(i4.4)') Nspec_x
+    write(e2, '(i4.4)') Nspec_z
+    write(e3, '(i4.4)') ny
+    write(e4, '(i5.5)') int(100d0*t)
+    path = trim(dirout)//'/'//name//'_'//e1//'x'//e2//'x'//e3//'_t'//e4//'.dat'
+    N = 0
+    N(1, 1:3) = Nspec_x
+    N(1, 4) = -2
+    N(2, 1:3) = Nspec_z
+    N(3, 3) = nyv
+    N(4, 3) = nyu
+    dummint = 0
+    allocate(buff(Nspec_x+2, Nspec_z))
+    open(10, file=path, form='unformatted', status='replace')
+    write(10) t, Re, alp, bet, mpgx, 3, iter, dummint
+    write(10) N
+    write(10) y, 1d0, dthdy
+    do j = j0, j1
+      do k = 1, Nspec_z
+        do i = 0, Nspec_x/2
+          col = (k-1)*(Nspec_x/2+1) + i + 1
+          buff(2*i+1, k) = dble(x(j, col))
+          buff(2*i+2, k) = dimag(x(j, col))
+        end do
+      end do
+      write(10) j, gridid, Nspec_x+2, Nspec_z, y(j), buff
+    end do
+    close(10)
+  end subroutine
+
+  subroutine write_fields(u1, u2, u3, p)
+    complex(8), intent(in) :: u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol)
+    real(8), allocatable :: yu(:), yv(:), du(:), dv(:)
+    allocate(yu(0:ny), yv(0:ny-1), du(0:ny), dv(0:ny-1))
+    call dns_hip_check(dns_ygrid(int(ny, c_int), dyq, int(ppp, c_int), yu, yv, du, dv), 'ygrid')
+    call write_one('u1', u1, 0, nyu+1, yu, du, 1)
+    call write_one('u2', u2, 0, nyv+1, yv, dv, 2)
+    call write_one('u3', u3, 0, nyu+1, yu, du, 3)
+    call write_one('p', p, 1, nyu, yu, du, 4)
+  end subroutine
+
+end program

@@ -1,0 +1,136 @@
+/* dns_hip.h -- C-ABI of the MI355X implementation of the channel-flow DNS RK3 time step.
+ *
+ * The reference has no plugin layer: its hot path is a set of external Fortran subroutines with
+ * implicit interfaces called from the main loop (littleharsh.f90:163-233), all state living in the
+ * `declaration` module.  This ABI exports one entry point per replaced subroutine plus a fused step;
+ * the Fortran interface module (final-single-block-_amd/fortran/dns_hip_mod.f90) binds them with
+ * ISO_C_BINDING and wraps them under the reference's subroutine names.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero status otherwise; dns_last_error() returns a
+ *    message for the calling thread.  The Fortran wrappers turn a non-zero status into `error stop`,
+ *    which keeps the reference's fatal `stop` semantics.
+ *  - Fields stay resident in GPU memory between calls.  Host arrays cross the ABI only in
+ *    dns_upload_fields / dns_download_fields, in the reference's column layout:
+ *       complex(8) x(j, column), j fastest, column = (k-1)*(Nx/2+1) + i + 1   (start.f90:963-981)
+ *       u1, u3: j = 0..nyu+1 ; u2: j = 0..nyv+1 ; p: j = 1..nyu        (littleharsh.f90:98-112)
+ *    passed as interleaved (re, im) doubles.
+ *  - One handle = one channel = one HIP stream.  Calls on one handle are not reentrant (the reference's
+ *    FFT work arrays make it non-reentrant too); different handles are independent.
+ */
+#ifndef DNS_HIP_H
+#define DNS_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNS_OK 0
+#define DNS_ERR_ARG 1     /* bad argument or unsupported size */
+#define DNS_ERR_HIP 2     /* HIP runtime error */
+#define DNS_ERR_STATE 3   /* call out of order (e.g. no fields uploaded) */
+#define DNS_ERR_NONFINITE 4 /* a NaN/Inf reached the CFL maxima */
+
+#define DNS_PREC_FP64 0   /* everything in FP64 */
+#define DNS_PREC_MIXED 1  /* FFTs, plane intermediates and products in FP32; solves, RK3 update in FP64 */
+
+typedef struct dns_handle dns_handle;
+
+/* Run parameters: the live keys of the reference's input.in (start.f90:86-252). */
+typedef struct dns_params {
+  int Mx, Mz, ny;         /* Ngalx1, Ngalz1 (physical points, 3/2-rule), ny (input.in:15-19) */
+  int ppp;                /* pp, polynomial exponent of the y grid (input.in:21) */
+  int flag_ctpress;       /* flpress: 1 constant pressure gradient, 0 constant flow rate (input.in:36) */
+  int precision;          /* DNS_PREC_FP64 or DNS_PREC_MIXED */
+  double Re, alp, bet;    /* Re, alp, bet (input.in:10-14) */
+  double CFL;             /* CFL (input.in:34) */
+  double mpgx;            /* mpgx (input.in:38) */
+  double dyq;             /* dyq (input.in:20) */
+  double fixed_dt;        /* > 0: use this dt instead of the reference's CFL rule (FOU3D.f90:290-303) */
+  int plane_batch;        /* planes per plane-stage launch group; 0 = all planes (autotuner sets it) */
+  int reserved[7];
+} dns_params;
+
+const char *dns_last_error(void);
+
+/* start.f90:53-663 for one rank: grid, wavenumbers, RK3 coefficients, dtv, weights; device buffers.
+ * Replaces `start` (start.f90:53); all set-up is host code, no kernel runs here. */
+int dns_create(const dns_params *prm, dns_handle **out);
+int dns_destroy(dns_handle *h);
+
+/* Sizes: out[0..9] = Nx, Nz, Kx, ncol, nyu, nyv, Mx, Mz, ny, device bytes allocated (MiB) */
+int dns_sizes(const dns_handle *h, long long *out);
+
+/* Host-side set-up arrays (ygrid, start.f90:665-806), no GPU needed:
+ * yu[ny+1], yv[ny], dthdyu[ny+1], dthdyv[ny], dyu2i[3*(ny+1)], dyv2i[3*ny] (3 coefficients per j),
+ * w[0..5] = dtv, gridweighting(1:2), gridweighting_bc_u1, gridweighting_interp(1:2). Any may be NULL. */
+int dns_setup_arrays(const dns_handle *h, double *yu, double *yv, double *dthdyu, double *dthdyv,
+                     double *dyu2i, double *dyv2i, double *w);
+
+/* ygrid (start.f90:665-806) without a handle or a GPU: yu[ny+1], yv[ny], dthdyu[ny+1], dthdyv[ny]. */
+int dns_ygrid(int ny, double dyq, int ppp, double *yu, double *yv, double *dthdyu, double *dthdyv);
+
+/* getini (start.f90:1221-1316) after the fields are in memory: uploads u1,u2,u3,p (reference column
+ * layout, host memory), sets t and iter, Qx, and QxT (QxT <= 0: QxT = flow rate of u1, as getini).
+ * The previous-substep nonlinear terms are cleared (they are multiplied by cRK(1) = 0 at kRK = 1). */
+int dns_upload_fields(dns_handle *h, const double *u1, const double *u2, const double *u3, const double *p,
+                      double t, int iter, double QxT);
+/* Copies the current fields back into host arrays in the reference layout (record_out's input). */
+int dns_download_fields(dns_handle *h, double *u1, double *u2, double *u3, double *p);
+/* Nonlinear terms Nu1 (j=1..nyu), Nu2 (j=1..nyv), Nu3 (j=1..nyu) in the reference layout. */
+int dns_download_nonlinear(dns_handle *h, double *Nu1, double *Nu2, double *Nu3);
+int dns_upload_nonlinear(dns_handle *h, const double *Nu1, const double *Nu2, const double *Nu3);
+/* Right-hand sides du1 (j=1..nyu), du2 (j=1..nyv), du3 (j=1..nyu) left by dns_rhs0 / dns_nonlinear. */
+int dns_download_rhs(dns_handle *h, double *du1, double *du2, double *du3);
+/* y-flux products uv_f (j=0..nyv+1), vv_c (j=0..nyu+1), wv_f (j=0..nyv+1) of the last dns_nonlinear. */
+int dns_download_flux(dns_handle *h, double *uv_f, double *vv_c, double *wv_f);
+/* psi and div (j = 1..nyu) of the last dns_solveP / dns_vcorr. */
+int dns_download_pressure_correction(dns_handle *h, double *psi, double *div);
+
+/* s[0..9] = t, dt, dti, mpgx, Qx, dgx, dtc, QxT, iter, max|div| (last dns_vcorr; 0 after fused steps) */
+int dns_get_scalars(dns_handle *h, double *s);
+/* Sets dt (and dti) for a call sequence that starts at kRK > 1 (the reference sets dt at kRK = 1). */
+int dns_set_dt(dns_handle *h, double dt);
+
+/* ---- one entry point per replaced subroutine (staged path, same arithmetic order) ---- */
+int dns_rhs0(dns_handle *h, int kRK);         /* RHS0_u1/u2/u3  littleharsh.f90:682-856 */
+int dns_nonlinear(dns_handle *h, int kRK);    /* nonlinear      FOU3D.f90:51-371 (+ dtc_calc at kRK=1) */
+int dns_solveU(dns_handle *h, int kRK);       /* solveU x3      littleharsh.f90:863-894, LUsolU tridLU_3D.f90:38 */
+int dns_meanflow_ctU(dns_handle *h, int kRK); /* meanflow_ctU   littleharsh.f90:591-607 (flag_ctpress==0) */
+int dns_solveP(dns_handle *h, int kRK);       /* solveP         littleharsh.f90:626-680, LUsolP tridLU_3D.f90:98 */
+int dns_vcorr(dns_handle *h, int kRK);        /* v_corr         littleharsh.f90:896-993 */
+int dns_meanflow_ctP(dns_handle *h);          /* meanflow_ctP   littleharsh.f90:572-589 (flag_ctpress==1) */
+
+/* ---- fused path ---- */
+/* One RK3 substep: RHS0 + nonlinear + solveU + meanflow_ctU + solveP + v_corr fused into the plane
+ * stage (3 kernels) and the column stage (3 sweeps).  littleharsh.f90:165-222 */
+int dns_rk3_substep(dns_handle *h, int kRK);
+/* nsteps full RK3 steps (littleharsh.f90:163-233), including meanflow_ctP and iter += 1. */
+int dns_rk3_step(dns_handle *h, int nsteps);
+/* Records one RK3 step (3 substeps) into a hipGraph on the handle's stream and replays it nsteps times. */
+int dns_rk3_step_graph(dns_handle *h, int nsteps);
+
+/* ---- lower-level entry points, on device memory owned by the caller ---- */
+/* four_to_phys_du then phys_to_four_du (FOU3D.f90:681-698, 1654-1672) on nplanes reference-format planes
+ * of (Mx+2) x Mz doubles (x fastest), in place on d_planes: in = spectral, out[0..n) = physical,
+ * and d_back (n planes) = back to spectral.  Planes use the handle's Mx, Mz, FP64. */
+int dns_plane_fft_roundtrip(dns_handle *h, const double *d_in, double *d_phys, double *d_back, int nplanes);
+/* modes_to_planes_UVP (FOU3D.f90:718-814) on device arrays: column field [ncol][nrows] (reference
+ * layout, complex) -> nrows reference planes (Mx+2) x Mz with the 3/2 zero padding. */
+int dns_modes_to_planes(dns_handle *h, const double *d_cols, int nrows, double *d_planes);
+/* planes_to_modes_UVP (FOU3D.f90:1694-1824): the inverse gather. */
+int dns_planes_to_modes(dns_handle *h, const double *d_planes, int nrows, double *d_cols);
+
+/* Waits for all work on the handle's stream. */
+int dns_sync(dns_handle *h);
+/* Average device duration (ms) per substep of each kernel, from HIP events on the handle's stream,
+ * over the dns_rk3_step calls made since dns_set_timing(h, 1): t[0] z-inverse, t[1] x-pass,
+ * t[2] z-forward, t[3] column sweep up, t[4] column sweep down, t[5] final sweep (+ mean-flow
+ * correction), t[6] whole substep.  With plane_batch below the plane count t[0] holds the whole plane
+ * stage and t[1], t[2] are 0.  n = number of substeps averaged. */
+int dns_set_timing(dns_handle *h, int on);
+int dns_get_timing(dns_handle *h, double *t, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

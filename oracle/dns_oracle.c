@@ -790,8 +790,10 @@ void orc_meanflow_ctU(orc_t *o, int kRK) {
   for (int j = 1; j <= o->nyv + 1; j++) rhs[j] = bdt;
   rhs[nu - 1] = 0.0;
   tri_solve(nu, a1, a2, a3, u11, rhs, w2, w3);
+  /* flowrateRe's dummy is u(1:nn+2) while u11 is allocated 0:nn+2 (start.f90:261), so by sequence
+   * association its u(j) is u11(j-1) (littleharsh.f90:505-511, 537) */
   double Qx1 = 0.0;
-  for (int j = 1; j <= o->nyv + 1; j++) Qx1 = Qx1 + (o->yv[j] - o->yv[j - 1]) * creal(u11[j]);
+  for (int j = 1; j <= o->nyv + 1; j++) Qx1 = Qx1 + (o->yv[j] - o->yv[j - 1]) * creal(u11[j - 1]);
   const double Qx0 = flowrate_re_part(o, o->u1);
   const double g = (o->QxT - Qx0) / Qx1;
   o->mpgx = o->mpgx + g;

@@ -176,6 +176,12 @@ def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+def time_reference_substeps(Mx, Mz, ny, u1, u2, u3, p, nsub=1, timeout=1800, **kw):
+    """Wall time (s) of ``nsub`` substeps of the reference's own hot path (mode 4 of the harness,
+    MPI_Wtime around the loop only; set-up and file I/O are outside the timed region)."""
+    return run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=4, nsteps=nsub, timeout=timeout, **kw)
+
+
 def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=0.0, it=0,
                   Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
                   flag_ctpress=1, QxT=0.0, Nu=None, fft_planes=None, timeout=3600):
@@ -207,6 +213,9 @@ def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=
         env.setdefault("OMP_NUM_THREADS", "1")
         subprocess.run([REF_HARNESS], cwd=d, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.PIPE, timeout=timeout, env=env)
+        if mode == 4:
+            with open(os.path.join(d, "timing.txt")) as f:
+                return float(f.read().split()[1])
         if mode == 3:
             out = np.fromfile(os.path.join(d, "fft_out.bin"), dtype=np.float64)
             n = fft_planes.shape[0]

@@ -25,7 +25,7 @@ program ref_harness
   include 'mpif.h'
   integer status(MPI_STATUS_SIZE), ierr, myid
   integer mode, ny_in, nsteps, istep, kRK_in, c, i, k, j, nplanes, ip
-  real(8) dt_in, QxT_in
+  real(8) dt_in, QxT_in, t_begin, t_end
   complex(8), allocatable :: u1(:,:), u2(:,:), u3(:,:), p(:,:), psi(:,:), div(:,:)
   complex(8), allocatable :: Nu1(:,:), Nu2(:,:), Nu3(:,:), du1(:,:), du2(:,:), du3(:,:)
   real(8), allocatable :: a_ugrid(:,:), a_vgrid(:,:), pl(:,:,:)
@@ -84,6 +84,30 @@ program ref_harness
   nstat = huge(nstat)
   nextqt = huge(nextqt)
   call divergence(div, u1, u2, u3, 0)
+
+  if (mode == 4) then
+    ! timing only: nsteps substeps (kRK = 1, 2, 3, 1, ...) of the main loop body, wall time of the loop
+    t_begin = MPI_Wtime()
+    do istep = 1, nsteps
+      kRK = mod(istep-1, 3) + 1
+      call RHS0_u1(du1, u1, Nu1, p, 0)
+      call RHS0_u2(du2, u2, Nu2, p, 0)
+      call RHS0_u3(du3, u3, Nu3, p, 0)
+      call nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, 0, status, ierr)
+      call solveU(u1, du1, ugrid, a_ugrid, 0)
+      call solveU(u2, du2, vgrid, a_vgrid, 0)
+      call solveU(u3, du3, ugrid, a_ugrid, 0)
+      if (flag_ctpress == 0) call meanflow_ctU(u1, 0)
+      call solveP(p, psi, u1, u2, u3, div, 0)
+      call v_corr(u1, u2, u3, psi, div, 0, status, ierr)
+    end do
+    t_end = MPI_Wtime()
+    open(49, file='timing.txt', form='formatted')
+    write(49,'(I6,ES24.15)') nsteps, t_end - t_begin
+    close(49)
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
 
   if (mode == 1) then
     open(43, file='hist.txt', form='formatted')

@@ -1,0 +1,111 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own compiled hot path.
+
+Run in a container that has the reference sources (oracle/_ref/ref_harness built by
+``make -C oracle ref``):  python tests/golden/make_golden.py
+
+Fixtures (inputs and expected outputs only):
+  golden_steps_24x24x17.npz : seeded state (seed 7) and the reference's fields after 10 RK3 steps,
+                              the per-step (t, dt) history; flag_ctpress = 1 and = 0 runs.
+  golden_stages_12x12x13.npz: one substep kRK = 1, 2, 3 with the stage-by-stage arrays
+                              (RHS0, nonlinear, fluxes, solveU, meanflow, solveP, v_corr).
+  golden_fft.npz            : random spectral planes through four_to_phys_du / phys_to_four_du.
+  golden_grid_153.npz       : y grid stored in the reference's 32x32x153 fixture files.
+  golden_fixture32_steps.npz: per-field norms and 8 columns of the reference run of 4 steps from its
+                              own 32x32x153 fixture (input/*_0032x0032x0153_t00000.dat).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "final-single-block-_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import dns_ic  # noqa: E402
+import dns_io  # noqa: E402
+import pyoracle  # noqa: E402
+
+REF_INPUT = "/root/reference/input"
+
+
+def main():
+    if not pyoracle.have_reference():
+        raise SystemExit("oracle/_ref/ref_harness is not built")
+    Mx = Mz = 24
+    ny = 17
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=7, amp=0.3, nmodes=4)
+    out = {"u1_in": st[0], "u2_in": st[1], "u3_in": st[2], "p_in": st[3]}
+    for ctp in (1, 0):
+        r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=10, flag_ctpress=ctp)
+        for k in ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3"):
+            out["%s_ctp%d" % (k, ctp)] = r[k]
+        out["hist_ctp%d" % ctp] = r["hist"]
+    np.savez_compressed(os.path.join(HERE, "golden_steps_24x24x17.npz"), **out)
+
+    # stage-by-stage for each kRK; kRK > 1 starts from the state after one full step, with the
+    # nonlinear terms and dt of that step
+    Mx = Mz = 12
+    ny = 13
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=5, amp=0.3, nmodes=3)
+    r1 = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=1)
+    base = [r1["u1"], r1["u2"], r1["u3"], r1["p"]]
+    Nu = [r1["Nu1"], r1["Nu2"], r1["Nu3"]]
+    out = {"u1_in": base[0], "u2_in": base[1], "u3_in": base[2], "p_in": base[3],
+           "Nu1_in": Nu[0], "Nu2_in": Nu[1], "Nu3_in": Nu[2], "dt_in": r1["dt"], "t_in": r1["t"]}
+    for k in (1, 2, 3):
+        for ctp in (1, 0):
+            r = pyoracle.run_reference(Mx, Mz, ny, *base, mode=2, kRK=k, dt=r1["dt"], t=r1["t"], Nu=Nu,
+                                       flag_ctpress=ctp)
+            tag = "k%d_ctp%d" % (k, ctp)
+            for q, n in enumerate(("du1", "du2", "du3")):
+                out["rhs0_%s_%s" % (n, tag)] = r["rhs0"][q]
+                out["dunl_%s_%s" % (n, tag)] = r["du_nl"][q]
+            for q, n in enumerate(("Nu1", "Nu2", "Nu3")):
+                out["nl_%s_%s" % (n, tag)] = r["Nu_new"][q]
+            for q, n in enumerate(("uv_f", "vv_c", "wv_f")):
+                out["flux_%s_%s" % (n, tag)] = r["flux"][q]
+            for q, n in enumerate(("u1", "u2", "u3")):
+                out["solveU_%s_%s" % (n, tag)] = r["u_solved"][q]
+                out["vcorr_%s_%s" % (n, tag)] = r["u_corr"][q]
+            out["meanflow_u1_%s" % tag] = r["u1_meanflow"]
+            out["solveP_p_%s" % tag] = r["p_new"]
+            out["solveP_psi_%s" % tag] = r["psi"]
+            out["vcorr_div_%s" % tag] = r["div"]
+            sc = r["stage_scalars"]
+            out["scalars_%s" % tag] = np.array([sc["dt"], sc["t"], sc["mpgx"], sc["dgx"], sc["dtc"]])
+    np.savez_compressed(os.path.join(HERE, "golden_stages_12x12x13.npz"), **out)
+
+    # FFT vectors: random retained spectral coefficients (reference plane layout), zero padding
+    out = {}
+    rng = np.random.default_rng(11)
+    for (mx, mz, n) in ((48, 48, 2), (24, 24, 2), (96, 96, 1)):
+        nx, nz = 2 * mx // 3, 2 * mz // 3
+        kx = nx // 2 + 1
+        pl = np.zeros((n, mz, mx + 2))
+        for s in range(n):
+            for zr in list(range(nz // 2)) + list(range(mz - nz // 2, mz)):
+                pl[s, zr, :2 * kx] = rng.standard_normal(2 * kx)
+        a, b = pyoracle.run_reference(mx, mz, 9, None, None, None, None, mode=3, fft_planes=pl)
+        out["in_%d" % mx], out["phys_%d" % mx], out["back_%d" % mx] = pl, a, b
+    np.savez_compressed(os.path.join(HERE, "golden_fft.npz"), **out)
+
+    if os.path.isdir(REF_INPUT):
+        ff = dns_io.read_field(os.path.join(REF_INPUT, "u1_0032x0032x0153_t00000.dat"))
+        fv = dns_io.read_field(os.path.join(REF_INPUT, "u2_0032x0032x0153_t00000.dat"))
+        np.savez_compressed(os.path.join(HERE, "golden_grid_153.npz"), yu=ff.y, dthdyu=ff.dthdy, yv=fv.y,
+                            dthdyv=fv.dthdy)
+        u1, u2, u3, p, head = dns_io.load_state(REF_INPUT, "_0032x0032x0153_t00000.dat", 153)
+        r = pyoracle.run_reference(48, 48, 153, u1, u2, u3, p, mode=1, nsteps=4, t=head.t)
+        cols = np.array([0, 1, 5, 17, 100, 271, 400, 543])
+        out = {"cols": cols, "hist": r["hist"]}
+        for k in ("u1", "u2", "u3", "p"):
+            out["norm_" + k] = np.linalg.norm(r[k])
+            out["cols_" + k] = r[k][cols]
+        np.savez_compressed(os.path.join(HERE, "golden_fixture32_steps.npz"), **out)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

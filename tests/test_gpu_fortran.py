@@ -1,0 +1,79 @@
+"""The Fortran drop-in driver (reference subroutine names over ISO_C_BINDING) on the GPU: reads
+reference-format input.in and field files, runs the reference's main loop, writes field files; the
+result is compared with the oracle."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import dns_ic
+import dns_io
+import dnship
+import pyoracle
+from conftest import PKG, rel_l2
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(PKG, "fortran", "littleharsh_hip")
+
+
+def write_case(d, Mx, Mz, ny, seed, flpress=1):
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=seed)
+    g = dnship.ygrid(ny)
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    N = np.zeros((4, 5), np.int32)
+    N[:, 3] = [Nx, Nz, ny - 2, ny - 1]
+    os.makedirs(os.path.join(d, "input"))
+    os.makedirs(os.path.join(d, "output"))
+    tag = "_%04dx%04dx%04d_t00000.dat" % (Nx, Nz, ny)
+    for name, f, j0, y, dth in (("u1", st[0], 0, g["yu"], g["dthdyu"]), ("u2", st[1], 0, g["yv"], g["dthdyv"]),
+                                ("u3", st[2], 0, g["yu"], g["dthdyu"]), ("p", st[3], 1, g["yu"], g["dthdyu"])):
+        planes = dns_io.columns_to_planes(f, Nz)
+        js = np.arange(j0, j0 + planes.shape[0])
+        ff = dns_io.FieldFile(0.0, 3250.0, 1.0, 2.0, -3.0675e-3, 3, 0, N, y, 1.0, dth, js, planes, y[js])
+        dns_io.write_field(os.path.join(d, "input", name + tag), ff)
+    with open(os.path.join(PKG, "fortran", "input.in")) as f:
+        lines = f.read().split("\n")
+    lines[14] = "Mx   = %10d" % Mx
+    lines[16] = "Mz   = %10d" % Mz
+    lines[18] = "ny   = %10d" % ny
+    lines[27] = "fnmeb = u1" + tag
+    lines[35] = "flpress    =%5d" % flpress
+    with open(os.path.join(d, "input.in"), "w") as f:
+        f.write("\n".join(lines))
+    return st, tag
+
+
+@pytest.mark.parametrize("mode", ["staged", "fused"])
+def test_fortran_driver_matches_oracle(tmp_path, mode):
+    assert os.path.exists(DRIVER), "build the Fortran driver first (__graft_entry__.build())"
+    Mx, Mz, ny, n = 48, 48, 33, 5
+    st, tag = write_case(str(tmp_path), Mx, Mz, ny, seed=12)
+    args = [DRIVER, str(n)] + (["fused"] if mode == "fused" else [])
+    print("case: fortran driver %s %dx%dx%d %d steps" % (mode, Mx, Mz, ny, n), flush=True)
+    r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    outs = os.listdir(os.path.join(str(tmp_path), "output"))
+    assert len(outs) == 4, outs
+    otag = outs[0][outs[0].index("_"):]
+    u1, u2, u3, p, head = dns_io.load_state(os.path.join(str(tmp_path), "output"), otag, ny)
+    o = pyoracle.Oracle(Mx, Mz, ny)
+    o.set_state(*st)
+    o.step(n)
+    for a, b in zip((u1, u2, u3, p), o.get_state()):
+        assert rel_l2(a, b) <= 1e-12
+    assert head.iter == n
+    assert abs(head.t - o.scalars()["t"]) <= 1e-13
+
+
+def test_fortran_driver_rejects_bad_size(tmp_path):
+    write_case(str(tmp_path), 48, 48, 33, seed=1)
+    with open(os.path.join(str(tmp_path), "input.in")) as f:
+        lines = f.read().split("\n")
+    lines[14] = "Mx   = %10d" % 60   # 60 = 2^2 3 5: FFTPACK's fatal factor
+    with open(os.path.join(str(tmp_path), "input.in"), "w") as f:
+        f.write("\n".join(lines))
+    r = subprocess.run([DRIVER, "1"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "prime factor" in r.stderr
