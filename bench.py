@@ -31,6 +31,8 @@ def gpu_leg(st, prec, steps, warmup):
     import dnship
     g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX, precision=prec)
     g.upload(*st)
+    chosen, ms = g.autotune(iters=3)   # tile variants of the plane kernels, timed on this workload
+    g.upload(*st)
     g.step(warmup)
     g.sync()
     g.set_timing(1)
@@ -42,7 +44,9 @@ def gpu_leg(st, prec, steps, warmup):
     sc = g.scalars()
     fields = g.download()
     g.close()
-    return (t1 - t0) / steps, tim, sc, fields
+    tune = {"kernels": list(dnship.DNS.KERNELS), "chosen": [int(c) for c in chosen],
+            "ms_per_variant": [[round(float(x), 4) for x in row] for row in ms]}
+    return (t1 - t0) / steps, tim, sc, fields, tune
 
 
 def cpu_leg(st):
@@ -93,7 +97,7 @@ def main():
     import dns_roofline
 
     st = dns_ic.seeded_channel(MX, MZ, NY, seed=3, amp=0.1, broadband=True)
-    sec64, tim64, sc64, f64 = gpu_leg(st, dnship.PREC_FP64, args.steps, args.warmup)
+    sec64, tim64, sc64, f64, tune64 = gpu_leg(st, dnship.PREC_FP64, args.steps, args.warmup)
     out = {"metric": METRIC, "value": 1.0 / sec64, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": sec64 * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64",
@@ -115,14 +119,16 @@ def main():
                       for k in KERNELS}
     out["step_frac_hbm"] = by["step"] / sec64 / 1e9 / dns_roofline.HBM_PEAK_GBS
     out["max_div"] = sc64["divmax"]
+    out["autotune"] = {"fp64": tune64}
     if not args.no_mixed:
-        secmx, timmx, scmx, fmx = gpu_leg(st, dnship.PREC_MIXED, args.steps, args.warmup)
+        secmx, timmx, scmx, fmx, tunemx = gpu_leg(st, dnship.PREC_MIXED, args.steps, args.warmup)
         err = {k: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for k, a, b in zip(("u1", "u2", "u3"), fmx, f64)}
         bym = dns_roofline.kernel_bytes(MX, MZ, NY, True)
         out["mixed"] = {"value": 1.0 / secmx, "ms_per_step": secmx * 1e3, "speedup_vs_fp64": sec64 / secmx,
                         "rel_l2_vs_fp64": err, "steps_compared": args.steps + args.warmup,
                         "kernels_ms": {k: timmx[k] for k in KERNELS},
                         "step_frac_hbm": bym["step"] / secmx / 1e9 / dns_roofline.HBM_PEAK_GBS}
+        out["autotune"]["mixed"] = tunemx
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(st)
     print(json.dumps(out), flush=True)

@@ -41,6 +41,7 @@ struct dns_handle {
   int Mx, Mz, Nx, Nz, Kx, C, ny, nyu, nyv;
   bool fp32;
   int batch;  // planes per plane-stage group
+  int var[5] = {0, 0, 0, 0, 0};  // tile variant per plane kernel (z-inverse, x-pass u/v, z-forward u/v)
   size_t bytes = 0;
   hipStream_t stream = nullptr;
   bool uploaded = false, timing = false;
@@ -57,6 +58,7 @@ struct dns_handle {
   double2 *twz64, *twx64, *twh64;  // FP64 twiddles for the reference-format plane utilities
   double2 *stage;                  // layout conversion buffer
   DnsState *st;
+  double *cfls = nullptr;  // CFL maxima spread over slots (plane stage at kRK = 1)
   std::vector<void *> allocs;
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev[8];
@@ -235,6 +237,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   rc |= chk(dalloc(h, &h->uvf, C * rv)); rc |= chk(dalloc(h, &h->wvf, C * rv)); rc |= chk(dalloc(h, &h->vvc, C * ru));
   rc |= chk(dalloc(h, &h->w3u, C * ru)); rc |= chk(dalloc(h, &h->w3v, C * rv)); rc |= chk(dalloc(h, &h->cP, C * rp));
   rc |= chk(dalloc(h, &h->mscr, 2 * ru));
+  rc |= chk(dalloc(h, &h->cfls, 2 * dnsk::CFL_SLOTS));
   rc |= chk(dalloc(h, &h->d_yu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_yv, (size_t)ny));
   rc |= chk(dalloc(h, &h->d_dthdyu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_dthdyv, (size_t)ny));
   rc |= chk(dalloc(h, &h->d_dyu2i, 3 * ((size_t)ny + 1))); rc |= chk(dalloc(h, &h->d_dyv2i, 3 * (size_t)ny));
@@ -375,7 +378,7 @@ static PlaneArgs plane_args(dns_handle *h, int kRK) {
   a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.yu = h->d_yu; a.yv = h->d_yv;
   a.I = h->I; a.P = h->P; a.twz = h->twz; a.twx = h->twx; a.twh = h->twh;
   a.Nxz1 = h->Nxz1; a.Nxz2 = h->Nxz2; a.Nxz3 = h->Nxz3; a.uvf = h->uvf; a.vvc = h->vvc; a.wvf = h->wvf;
-  a.cfl = &h->st->cfl[0];
+  a.cfl = h->cfls;
   return a;
 }
 
@@ -394,26 +397,34 @@ static void tmark(dns_handle *h, int q) {
 static int plane_stage(dns_handle *h, int kRK) {
   if (kRK == 1) {
     dnsk::launch_zero(&h->st->cfl[0], 3, h->stream);  // cfl[0], cfl[1], divmax
-    dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, &h->st->cfl[0], h->stream);
+    dnsk::launch_zero(h->cfls, 2 * dnsk::CFL_SLOTS, h->stream);
+    dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, h->cfls, h->stream);
   }
   PlaneArgs a = plane_args(h, kRK);
   const int total = 2 * h->nyu;
   const bool one_group = h->batch >= total;  // per-kernel event marks only with one launch per kernel
   tmark(h, 0);
-  for (int w0 = 0; w0 < total; w0 += h->batch) {
-    const int nw = std::min(h->batch, total - w0);
-    a.w0 = w0;
-    for (int which = 0; which < 3; which++) {
-      if (dnsk::launch_plane_stage_split(a, nw, h->fp32, h->stream, which))
-        return fail(DNS_ERR_ARG, "plane stage: sizes %d x %d not instantiated", h->Mx, h->Mz);
-      if (one_group) tmark(h, which + 1);
+  for (int b0 = 0; b0 < total; b0 += h->batch) {
+    const int b1 = std::min(b0 + h->batch, total);
+    const int um = std::min(b1, h->nyu), vm = std::max(b0, h->nyu);  // u-planes [b0, um), v-planes [vm, b1)
+    a.wbase = b0;
+    int rc = 0;
+    a.w0 = b0;
+    rc |= dnsk::launch_plane_kernel(a, b1 - b0, h->fp32, 0, h->var[0], h->stream);
+    if (one_group) tmark(h, 1);
+    for (int k = 0; k < 2; k++) {  // x-pass, then z-forward
+      if (um > b0) { a.w0 = b0; rc |= dnsk::launch_plane_kernel(a, um - b0, h->fp32, 1 + 2 * k, h->var[1 + 2 * k], h->stream); }
+      if (b1 > vm) { a.w0 = vm; rc |= dnsk::launch_plane_kernel(a, b1 - vm, h->fp32, 2 + 2 * k, h->var[2 + 2 * k], h->stream); }
+      if (one_group) tmark(h, 2 + k);
     }
+    if (rc) return fail(DNS_ERR_ARG, "plane stage: sizes %d x %d not instantiated", h->Mx, h->Mz);
   }
   if (!one_group) {
     tmark(h, 1);
     tmark(h, 2);
     tmark(h, 3);
   }
+  if (kRK == 1) dnsk::launch_cfl_reduce(h->cfls, &h->st->cfl[0], h->stream);
   HIPCHK(hipGetLastError());
   return DNS_OK;
 }
@@ -663,6 +674,84 @@ extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
   }
   for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(h->graph, h->stream));
   return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Plane-kernel tile variants.  Every variant of a kernel gives bit-identical results (same butterflies,
+// same order of operations per element); they differ in planes/rows per workgroup and threads.
+static int plane_len(const dns_handle *h, int which) { return (which == 1 || which == 2) ? h->Mx : h->Mz; }
+
+static void drop_graph(dns_handle *h) {
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  h->graph = nullptr;
+}
+
+extern "C" int dns_get_plane_variants(dns_handle *h, int *var, int *count) {
+  if (!h || !var) return fail(DNS_ERR_ARG, "dns_get_plane_variants: null argument");
+  for (int c = 0; c < 5; c++) {
+    var[c] = h->var[c];
+    if (count) count[c] = dnsk::plane_variants(plane_len(h, c));
+  }
+  return DNS_OK;
+}
+
+extern "C" int dns_set_plane_variants(dns_handle *h, const int *var) {
+  if (!h || !var) return fail(DNS_ERR_ARG, "dns_set_plane_variants: null argument");
+  for (int c = 0; c < 5; c++)
+    if (var[c] < 0 || var[c] >= dnsk::plane_variants(plane_len(h, c)))
+      return fail(DNS_ERR_ARG, "dns_set_plane_variants: kernel %d has no variant %d (length %d has %d)", c, var[c],
+                  plane_len(h, c), dnsk::plane_variants(plane_len(h, c)));
+  for (int c = 0; c < 5; c++) h->var[c] = var[c];
+  drop_graph(h);
+  return DNS_OK;
+}
+
+// Times every variant of every plane kernel on the first plane batch (kRK = 2 arguments: no CFL maxima)
+// and keeps the fastest.  The plane-stage outputs are scratch that the next substep rewrites, so this
+// may be called before a step or between steps, not between the staged calls of one substep.
+extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_autotune: null handle");
+  if (iters < 1 || iters > 1000) return fail(DNS_ERR_ARG, "dns_autotune: iters=%d not in 1..1000", iters);
+  if (!h->uploaded) return fail(DNS_ERR_STATE, "dns_autotune: no fields uploaded");
+  PlaneArgs a = plane_args(h, 2);
+  const int total = 2 * h->nyu, b1 = std::min(h->batch, total);
+  const int um = std::min(b1, h->nyu);
+  a.wbase = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = DNS_OK;
+  for (int c = 0; c < 5 && rc == DNS_OK; c++) {
+    int w0 = 0, nw = b1;  // z-inverse: the whole batch
+    if (c == 1 || c == 3) nw = um;
+    if (c == 2 || c == 4) { w0 = h->nyu; nw = b1 - h->nyu; }
+    const int nv = dnsk::plane_variants(plane_len(h, c));
+    int best = 0;
+    double tbest = 1e300;
+    for (int v = 0; v < 3; v++) {
+      if (ms) ms[3 * c + v] = -1.0;
+      if (v >= nv || nw <= 0) continue;
+      a.w0 = w0;
+      if (dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream)) { rc = fail(DNS_ERR_ARG, "dns_autotune: size not instantiated"); break; }
+      hipError_t e = hipEventRecord(e0, h->stream);
+      for (int it = 0; it < iters && e == hipSuccess; it++) dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) { rc = fail(DNS_ERR_HIP, "dns_autotune: %s", hipGetErrorString(e)); break; }
+      const double tv = (double)t / iters;
+      if (ms) ms[3 * c + v] = tv;
+      if (tv < tbest) { tbest = tv; best = v; }
+    }
+    h->var[c] = best;
+    if (chosen) chosen[c] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  drop_graph(h);
+  if (rc == DNS_OK) HIPCHK(hipGetLastError());
+  return rc;
 }
 
 extern "C" int dns_sync(dns_handle *h) {

@@ -10,16 +10,19 @@ namespace dnsk {
 //   u1, u3, du1, du3, vvc : rows 0..nyu+1 at j*C        u2, du2, uvf, wvf : rows 0..nyv+1 at j*C
 //   p, psi, div, Nu1, Nu3, Nxz1, Nxz3 : rows 1..nyu at (j-1)*C
 //   Nu2, Nxz2 : rows 1..nyv at (j-1)*C
+constexpr int CFL_SLOTS = 64;  // addresses the CFL maxima are spread over
+
 struct PlaneArgs {
   int Mx, Mz, Nx, Nz, Kx, C, nyu, nyv;
-  int w0, kRK;
+  int w0, kRK;                     // first plane of this launch; RK substep
+  int wbase;                       // first plane held in I/P (the plane batch start)
   double alp, bet;
   const double2 *u1, *u2, *u3;
   const double *yu, *yv;
   void *I, *P;                     // plane intermediates, precision of the plane stage
   const void *twz, *twx, *twh;     // twiddles e^{-2 pi i m/N} for N = Mz, Mx, Mx/2
   double2 *Nxz1, *Nxz2, *Nxz3, *uvf, *vvc, *wvf;
-  double *cfl;                     // [0] max|u1|, [1] max|u3|
+  double *cfl;                     // CFL_SLOTS partial max|u1|, then CFL_SLOTS partial max|u3|
 };
 
 struct DnsState {
@@ -48,9 +51,11 @@ struct ColArgs {
 };
 
 bool plane_size_supported(int n);
-int xpass_rows_per_block(int Mx);  // rows of one x-pass block; must divide Mz
-int launch_plane_stage(const PlaneArgs &a, int nw, bool fp32, hipStream_t st);
-int launch_plane_stage_split(const PlaneArgs &a, int nw, bool fp32, hipStream_t st, int which);
+int xpass_rows_per_block(int Mx);  // rows of one x-pass block (largest variant); must divide Mz
+int plane_variants(int n);          // tile variants instantiated for transform length n
+// which: 0 z-inverse (any planes), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; planes [a.w0, a.w0+nw)
+int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);
+void launch_cfl_reduce(const double *slots, double *out, hipStream_t st);
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st);
 void launch_zero(double *p, int n, hipStream_t st);
 int launch_plane_roundtrip(int Mx, int Mz, const double *in, double *phys, double *back, int n,

@@ -1,0 +1,239 @@
+// Plane-stage kernels of the nonlinear term (FOU3D.f90:51-371, ops_in_planes2 FOU3D.f90:1514-1652).
+//
+// Per y-plane the reference runs: zero-padded inverse FFT in z on the Kx retained x-modes (cft, pruned),
+// inverse real FFT in x on all Mz rows (rft), pointwise products, forward real FFT in x, forward FFT in z
+// on the Kx x-modes, the spectral derivatives der_x/der_z with their dealias mask, and keeps vv, uv, wv
+// for the y-flux.  Here that work is three kernels over a batch of planes:
+//   zinv_kernel : spectral plane (device layout [row][column]) -> I[plane][field][z][i]   (z-inverse)
+//                 The y-interpolations interp_u/interp_v (FOU3D.f90:373-418) are fused into its loads.
+//   xpass_kernel: I rows -> physical rows -> products -> P[plane][product][z][i]          (x-pass)
+//                 Also the CFL maxima of dtc_calc (FOU3D.f90:613-651) at kRK = 1.
+//   zfwd_kernel : P -> forward z FFT -> der_x + der_z (masked) and raw fluxes in device layout.
+// Plane index w: w < nyu is u-plane j = w+1 (u1, u2_itp, u3 -> uu, uw, vv, ww); otherwise v-plane
+// j = w-nyu+1 (u1_itp, u2, u3_itp -> uv, wv).  v-planes run to j = nyv+1 as the reference does on one
+// rank (start.f90:1128-1180 gives the single rank v-planes 1..nyu).
+#pragma once
+#include "dns_kernels.h"
+
+namespace dnsk {
+
+// stride N / first radix of plan PL for length N
+template <class PL, int N> constexpr int PL_M = N / plan_info<PL>::first;
+
+// ------------------------------------------------------------------------------------------------
+template <typename R, int MZ, int T, int NTH, class PL, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
+  using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
+  __shared__ cx<R> s[L::SIZE];
+  const int w = a.w0 + blockIdx.y;
+  const int i0 = blockIdx.x * T;
+  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
+  const bool vpl = w >= a.nyu;
+  const int j = vpl ? w - a.nyu + 1 : w + 1;
+  const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
+  cx<R> *I = reinterpret_cast<cx<R> *>(a.I) + (size_t)(w - a.wbase) * 3 * MZ * Kx;
+  // interpolation weights (interp_v for u-planes, interp_u for v-planes): wa/dd on the upper plane,
+  // wb/dd on the lower one
+  double ca, cb;
+  if (!vpl) {
+    const double dd = a.yv[j] - a.yv[j - 1];
+    ca = (a.yu[j] - a.yv[j - 1]) / dd;
+    cb = (a.yv[j] - a.yu[j]) / dd;
+  } else {
+    const double dd = a.yu[j + 1] - a.yu[j];
+    ca = (a.yv[j] - a.yu[j]) / dd;
+    cb = (a.yu[j + 1] - a.yv[j]) / dd;
+  }
+  const int f = blockIdx.z;  // one field per block
+  {
+    // field f: on its own plane (hi only) or interpolated between planes lo and hi
+    const bool itp = vpl ? (f != 1) : (f == 1);
+    const double2 *fld = f == 0 ? a.u1 : f == 1 ? a.u2 : a.u3;
+    const double2 *hi = fld + (size_t)(vpl && itp ? j + 1 : j) * C;
+    const double2 *lo = fld + (size_t)(vpl ? j : j - 1) * C;
+    // position i + rM of column i0 + t; the rows of the 3/2 padding band (MZ/3 <= z < 2 MZ/3) are zero.
+    // When the first pass's stride divides MZ/3 the band is known per rM at compile time (no load).
+    auto ld = [&](int i, auto rM, int t) -> cx<R> {
+      constexpr int zr = decltype(rM)::value, B0 = MZ / 3, B1 = 2 * MZ / 3;
+      constexpr int M = PL_M<PL, MZ>;
+      constexpr bool exact = (B0 % M) == 0;
+      const int z = i + zr;
+      const bool kept = exact ? (zr < B0 || zr >= B1) : (z < B0 || z >= B1);
+      if (exact && !kept) return mk<R>(0, 0);
+      const int ic = min(i0 + t, Kx - 1);             // clamped: loads stay in bounds, masked below
+      const int zc = (exact ? zr < B0 : z < B0) ? z : z - B0;
+      const int col = zc * Kx + ic;
+      const double2 x1 = hi[col];
+      double2 v = x1;
+      if (itp) {
+        const double2 x0 = lo[col];
+        v = d2(ca * x1.x + cb * x0.x, ca * x1.y + cb * x0.y);
+      }
+      if (!kept || i0 + t >= Kx) v = d2(0.0, 0.0);
+      return from_d2<R>(v);
+    };
+    cx<R> *If = I + (size_t)f * MZ * Kx + i0;
+    auto st = [&](int z, int t, const cx<R> &v) {
+      if (i0 + t < Kx) If[z * Kx + t] = v;
+    };
+    fft_mem<R, L, NTH, +1, PL>(s, tw, ld, st);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// x-pass.  A real row x of length MX is packed as z_m = x_{2m} + i x_{2m+1} (H = MX/2 complex), so one
+// length-H complex FFT does each real transform.  Inverse (rft_buff.f:76-84 semantics: Im c_0 and
+// Im c_{MX/2} ignored, unnormalised Hermitian synthesis):
+//   Z_n = (X_n + conj X_{H-n}) + i e^{+2 pi i n/MX} (X_n - conj X_{H-n}),  z = IFFT_H(Z)
+// Forward (rft_buff.f:62-74: scaled 1/MX, Im c_0 = 0):
+//   X_k = [ (Z_k + conj Z_{H-k}) - i e^{-2 pi i k/MX} (Z_k - conj Z_{H-k}) ] / (2 MX),  Z = FFT_H(z)
+// Each row owns 4 LDS slots: the 3 velocities go to slots 0..2, the products are formed in place
+// (u-planes: uu, uw, vv, ww in slots 0..3; v-planes: uv, wv in slots 0..1).  VPL selects the plane type;
+// the launch covers planes of one type only.
+template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
+  constexpr int H = MX / 2;
+  constexpr int NS = RB * 4;
+  constexpr int NP = VPL ? 2 : 4;
+  using LI = Lay<H, NS, RB * 3, 3, 4, PS, true>;   // inverse: slots r*4 + f, f < 3
+  using LF = Lay<H, NS, RB * NP, NP, 4, PS, true>;  // forward: slots r*4 + p, p < NP
+  __shared__ cx<R> A[LI::SIZE];
+  __shared__ double red[NTH / 64];
+  const int w = a.w0 + blockIdx.y;
+  const int z0 = blockIdx.x * RB;
+  const int Kx = a.Kx, MZ = a.Mz;
+  const int j = VPL ? w - a.nyu + 1 : w + 1;
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
+  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}
+  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + (size_t)(w - a.wbase) * 3 * MZ * Kx;
+  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + (size_t)(w - a.wbase) * 4 * MZ * Kx;
+
+  // packed half-length spectrum Z_n of field f of row z0 + r (active transform q = r*3 + f)
+  auto ld = [&](int i, auto rM, int q) -> cx<R> {
+    const int n = i + decltype(rM)::value;
+    const int r = q / 3, f = q - r * 3;
+    const cx<R> *row = I + ((size_t)f * MZ + (z0 + r)) * Kx;
+    cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
+    if (n < Kx) xa = row[n];
+    if (n == 0) xa.y = 0;
+    if (n > 0 && H - n < Kx) xb = row[H - n];
+    const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
+    const cx<R> e = cconj(twx[n]);                        // e^{+2 pi i n/MX}
+    return cadd(sum, cmul_si<1>(cmul(dif, e)));
+  };
+  fft_from_mem<R, LI, NTH, +1, PLI>(A, twh, ld);
+
+  R m1 = 0, m3 = 0;
+  const bool cfl = (a.kRK == 1) && !VPL && (j <= a.nyu - 1);
+  for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
+    const int r = idx / H, n = idx - r * H;
+    const int e = LI::row(n);
+    cx<R> *b0 = A + (r * 4) * LI::LR + e;
+    const cx<R> zu = b0[0], zv = b0[LI::LR], zw = b0[2 * LI::LR];
+    if (!VPL) {
+      b0[0] = mk<R>(zu.x * zu.x, zu.y * zu.y);                // uu
+      b0[LI::LR] = mk<R>(zu.x * zw.x, zu.y * zw.y);           // uw
+      b0[2 * LI::LR] = mk<R>(zv.x * zv.x, zv.y * zv.y);       // vv
+      b0[3 * LI::LR] = mk<R>(zw.x * zw.x, zw.y * zw.y);       // ww
+      if (cfl) {
+        m1 = fmax(m1, fmax(fabs(zu.x), fabs(zu.y)));
+        m3 = fmax(m3, fmax(fabs(zw.x), fabs(zw.y)));
+      }
+    } else {
+      b0[0] = mk<R>(zu.x * zv.x, zu.y * zv.y);                // uv
+      b0[LI::LR] = mk<R>(zw.x * zv.x, zw.y * zv.y);           // wv
+    }
+  }
+  if (cfl) {
+    const double b1 = block_max<NTH>((double)m1, red);
+    const double b3 = block_max<NTH>((double)m3, red);
+    if (threadIdx.x == 0) {  // spread over CFL_SLOTS addresses: one address for every block serialises
+      const int slot = (blockIdx.x + blockIdx.y) & (CFL_SLOTS - 1);
+      atomic_max_nonneg(&a.cfl[slot], b1);
+      atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);
+    }
+  }
+  __syncthreads();
+  fft_lds<R, LF, NTH, -1, PLF>(A, twh);
+
+  const R sc = (R)(0.5 / (double)MX);
+  for (int idx = threadIdx.x; idx < Kx * RB * NP; idx += NTH) {
+    const int q = idx / Kx, k = idx - q * Kx;
+    const int r = q / NP, p = q - r * NP;
+    const cx<R> *b0 = A + (r * 4 + p) * LF::LR;
+    const cx<R> zk = b0[LF::row(k)];
+    const cx<R> zm = cconj(b0[LF::row((H - k) % H)]);
+    const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
+    cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
+    x = cscale(x, sc);
+    if (k == 0) x.y = 0;
+    P[((size_t)p * MZ + (z0 + r)) * Kx + k] = x;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// The product fields are transformed one at a time, ordered so that at most two outputs are live in
+// registers: u-planes uw (starts der_z(uw) and der_x(uw)), uu (completes Nxz1 = der_x(uu) + der_z(uw)),
+// ww (completes Nxz3 = der_x(uw) + der_z(ww)), vv (flux); v-planes uv (flux, starts der_x(uv)),
+// wv (flux, completes Nxz2 = der_x(uv) + der_z(wv)).  Sums are formed in the reference's operand order
+// (ops_in_planes2, FOU3D.f90:1590-1640).  The kz = -Nz/2 row is inside der_x/der_z's zero band; the raw
+// fluxes keep it.
+template <typename R, int MZ, int T, int NTH, bool VPL, class PL, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
+  using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
+  __shared__ cx<R> s[L::SIZE];
+  constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
+  constexpr int NP = VPL ? 2 : 4;
+  const int w = a.w0 + blockIdx.y;
+  const int i0 = blockIdx.x * T;
+  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
+  const int j = VPL ? w - a.nyu + 1 : w + 1;
+  const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
+  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)(w - a.wbase) * 4 * MZ * Kx;
+  const double inv = 1.0 / (double)MZ;
+  const size_t ju = (size_t)(j - 1) * C, jf = (size_t)j * C;
+  double2 accA[PERMAX], accB[PERMAX];
+#pragma unroll
+  for (int step = 0; step < NP; step++) {
+    // product slot of this step: u-planes uw, uu, ww, vv = 1, 0, 3, 2; v-planes uv, wv = 0, 1
+    const int p = VPL ? step : (step == 0 ? 1 : step == 1 ? 0 : step == 2 ? 3 : 2);
+    auto ld = [&](int i, auto rM, int t) -> cx<R> {
+      const int z = i + decltype(rM)::value;
+      const int ic = min(i0 + t, Kx - 1);             // clamped: loads stay in bounds, masked below
+      const cx<R> v = P[((size_t)p * MZ + z) * Kx + ic];
+      return i0 + t < Kx ? v : mk<R>(0, 0);
+    };
+    fft_from_mem<R, L, NTH, -1, PL>(s, tw, ld);
+#pragma unroll
+    for (int q = 0; q < PERMAX; q++) {
+      const int idx = threadIdx.x + q * NTH;
+      if (idx < Nz * T) {
+        const int k = idx / T, t = idx - k * T, i = i0 + t;
+        const int zp = k < Nz / 2 ? k : k + MZ - Nz;
+        double2 v = to_d2(s[L::addr(zp, t)]);
+        v = d2(inv * v.x, inv * v.y);
+        const double kx = a.alp * (double)i;
+        const double kz = k < Nz / 2 ? a.bet * (double)k : -(a.bet * (double)(Nz - k));
+        const bool st = i < Kx, masked = (k == Nz / 2);
+        const size_t col = (size_t)k * Kx + i;
+        if (!VPL) {
+          if (step == 0) { accA[q] = imul(kz, v); accB[q] = imul(kx, v); }
+          else if (step == 1) { if (st) a.Nxz1[ju + col] = masked ? d2(0.0, 0.0) : cadd(imul(kx, v), accA[q]); }
+          else if (step == 2) { if (st) a.Nxz3[ju + col] = masked ? d2(0.0, 0.0) : cadd(accB[q], imul(kz, v)); }
+          else { if (st) a.vvc[jf + col] = v; }
+        } else {
+          if (step == 0) { accA[q] = imul(kx, v); if (st) a.uvf[jf + col] = v; }
+          else {
+            if (st) {
+              if (j <= a.nyv) a.Nxz2[ju + col] = masked ? d2(0.0, 0.0) : cadd(accA[q], imul(kz, v));
+              a.wvf[jf + col] = v;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace dnsk

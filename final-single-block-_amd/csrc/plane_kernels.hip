@@ -1,232 +1,8 @@
-// Plane stage of the nonlinear term (FOU3D.f90:51-371, ops_in_planes2 FOU3D.f90:1514-1652).
-//
-// Per y-plane the reference runs: zero-padded inverse FFT in z on the Kx retained x-modes (cft, pruned),
-// inverse real FFT in x on all Mz rows (rft), pointwise products, forward real FFT in x, forward FFT in z
-// on the Kx x-modes, the spectral derivatives der_x/der_z with their dealias mask, and keeps vv, uv, wv
-// for the y-flux.  Here that work is three kernels over a batch of planes:
-//   zinv_kernel : spectral plane (device layout [row][column]) -> I[plane][field][z][i]   (z-inverse)
-//                 The y-interpolations interp_u/interp_v (FOU3D.f90:373-418) are fused into its loads.
-//   xpass_kernel: I rows -> physical rows -> products -> P[plane][product][z][i]          (x-pass)
-//                 Also the CFL maxima of dtc_calc (FOU3D.f90:613-651) at kRK = 1.
-//   zfwd_kernel : P -> forward z FFT -> der_x + der_z (masked) and raw fluxes in device layout.
-// Plane index w: w < nyu is u-plane j = w+1 (u1, u2_itp, u3 -> uu, uw, vv, ww); otherwise v-plane
-// j = w-nyu+1 (u1_itp, u2, u3_itp -> uv, wv).  v-planes run to j = nyv+1 as the reference does on one
-// rank (start.f90:1128-1180 gives the single rank v-planes 1..nyu).
-#include "dns_kernels.h"
+// Plane stage of the nonlinear term: CFL helpers, the reference-format plane round trip, and the
+// launchers of the plane kernels (plane_kernels.h) for every supported transform length.
+#include "plane_kernels.h"
 
 namespace dnsk {
-
-// ------------------------------------------------------------------------------------------------
-template <typename R, int MZ, int T, int NTH>
-__global__ void __launch_bounds__(NTH) zinv_kernel(PlaneArgs a) {
-  __shared__ cx<R> s[MZ * T];
-  const int w = a.w0 + blockIdx.y;
-  const int i0 = blockIdx.x * T;
-  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
-  const bool vpl = w >= a.nyu;
-  const int j = vpl ? w - a.nyu + 1 : w + 1;
-  const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
-  cx<R> *I = reinterpret_cast<cx<R> *>(a.I) + (size_t)(w - a.w0) * 3 * MZ * Kx;
-  // interpolation weights (interp_v for u-planes, interp_u for v-planes)
-  double wa, wb, dd;
-  if (!vpl) {
-    wa = a.yu[j] - a.yv[j - 1];
-    wb = a.yv[j] - a.yu[j];
-    dd = a.yv[j] - a.yv[j - 1];
-  } else {
-    wa = a.yv[j] - a.yu[j];
-    wb = a.yu[j + 1] - a.yv[j];
-    dd = a.yu[j + 1] - a.yu[j];
-  }
-  for (int f = 0; f < 3; f++) {
-    for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
-      const int z = idx / T, t = idx - z * T, i = i0 + t;
-      double2 v = d2(0.0, 0.0);
-      const bool kept = (z < Nz / 2) || (z >= MZ - Nz / 2);
-      if (kept && i < Kx) {
-        const int k = z < Nz / 2 ? z : z - (MZ - Nz);
-        const size_t col = (size_t)k * Kx + i;
-        if (!vpl) {
-          if (f == 0) v = a.u1[(size_t)j * C + col];
-          else if (f == 2) v = a.u3[(size_t)j * C + col];
-          else {
-            const double2 x1 = a.u2[(size_t)j * C + col], x0 = a.u2[(size_t)(j - 1) * C + col];
-            v = d2((wa * x1.x + wb * x0.x) / dd, (wa * x1.y + wb * x0.y) / dd);
-          }
-        } else {
-          if (f == 1) v = a.u2[(size_t)j * C + col];
-          else {
-            const double2 *u = f == 0 ? a.u1 : a.u3;
-            const double2 x1 = u[(size_t)(j + 1) * C + col], x0 = u[(size_t)j * C + col];
-            v = d2((wa * x1.x + wb * x0.x) / dd, (wa * x1.y + wb * x0.y) / dd);
-          }
-        }
-      }
-      s[idx] = from_d2<R>(v);
-    }
-    __syncthreads();
-    fft_lds<R, MZ, T, NTH, +1>(s, tw);
-    for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
-      const int z = idx / T, t = idx - z * T, i = i0 + t;
-      if (i < Kx) I[((size_t)f * MZ + z) * Kx + i] = s[idx];
-    }
-    __syncthreads();
-  }
-}
-
-// ------------------------------------------------------------------------------------------------
-// x-pass.  A real row x of length MX is packed as z_m = x_{2m} + i x_{2m+1} (H = MX/2 complex), so one
-// length-H complex FFT does each real transform.  Inverse (rft_buff.f:76-84 semantics: Im c_0 and
-// Im c_{MX/2} ignored, unnormalised Hermitian synthesis):
-//   Z_n = (X_n + conj X_{H-n}) + i e^{+2 pi i n/MX} (X_n - conj X_{H-n}),  z = IFFT_H(Z)
-// Forward (rft_buff.f:62-74: scaled 1/MX, Im c_0 = 0):
-//   X_k = [ (Z_k + conj Z_{H-k}) - i e^{-2 pi i k/MX} (Z_k - conj Z_{H-k}) ] / (2 MX),  Z = FFT_H(z)
-template <typename R, int MX, int RB, int NTH>
-__global__ void __launch_bounds__(NTH) xpass_kernel(PlaneArgs a) {
-  constexpr int H = MX / 2;
-  constexpr int NA = RB * 3, NB = RB * 4;
-  __shared__ cx<R> A[H * NA];
-  __shared__ cx<R> B[H * NB];
-  __shared__ double red[NTH / 64];
-  const int w = a.w0 + blockIdx.y;
-  const int z0 = blockIdx.x * RB;
-  const int Kx = a.Kx, MZ = a.Mz;
-  const bool vpl = w >= a.nyu;
-  const int j = vpl ? w - a.nyu + 1 : w + 1;
-  const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
-  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}
-  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + (size_t)(w - a.w0) * 3 * MZ * Kx;
-  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + (size_t)(w - a.w0) * 4 * MZ * Kx;
-
-  for (int idx = threadIdx.x; idx < H * NA; idx += NTH) {
-    const int slot = idx / H, n = idx - slot * H;
-    const int r = slot / 3, f = slot - r * 3;
-    const cx<R> *row = I + ((size_t)f * MZ + (z0 + r)) * Kx;
-    cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
-    if (n < Kx) xa = row[n];
-    if (n == 0) xa.y = 0;
-    if (n > 0 && H - n < Kx) xb = row[H - n];
-    const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
-    const cx<R> e = cconj(twx[n]);                        // e^{+2 pi i n/MX}
-    A[n * NA + slot] = cadd(sum, cmul_si<1>(cmul(dif, e)));
-  }
-  __syncthreads();
-  fft_lds<R, H, NA, NTH, +1>(A, twh);
-
-  R m1 = 0, m3 = 0;
-  const bool cfl = (a.kRK == 1) && !vpl && (j <= a.nyu - 1);
-  for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
-    const int r = idx / H, n = idx - r * H;
-    const cx<R> zu = A[n * NA + r * 3 + 0], zv = A[n * NA + r * 3 + 1], zw = A[n * NA + r * 3 + 2];
-    cx<R> *b = B + n * NB + r * 4;
-    if (!vpl) {
-      b[0] = mk<R>(zu.x * zu.x, zu.y * zu.y);  // uu
-      b[1] = mk<R>(zu.x * zw.x, zu.y * zw.y);  // uw
-      b[2] = mk<R>(zv.x * zv.x, zv.y * zv.y);  // vv
-      b[3] = mk<R>(zw.x * zw.x, zw.y * zw.y);  // ww
-      if (cfl) {
-        m1 = fmax(m1, fmax(fabs(zu.x), fabs(zu.y)));
-        m3 = fmax(m3, fmax(fabs(zw.x), fabs(zw.y)));
-      }
-    } else {
-      b[0] = mk<R>(zu.x * zv.x, zu.y * zv.y);  // uv
-      b[1] = mk<R>(zw.x * zv.x, zw.y * zv.y);  // wv
-      b[2] = mk<R>(0, 0);
-      b[3] = mk<R>(0, 0);
-    }
-  }
-  if (cfl) {
-    const double b1 = block_max<NTH>((double)m1, red);
-    const double b3 = block_max<NTH>((double)m3, red);
-    if (threadIdx.x == 0) {
-      atomic_max_nonneg(&a.cfl[0], b1);
-      atomic_max_nonneg(&a.cfl[1], b3);
-    }
-  }
-  __syncthreads();
-  fft_lds<R, H, NB, NTH, -1>(B, twh);
-
-  const int np = vpl ? 2 : 4;
-  const R sc = (R)(0.5 / (double)MX);
-  for (int idx = threadIdx.x; idx < Kx * RB * np; idx += NTH) {
-    const int q = idx / Kx, k = idx - q * Kx;
-    const int r = q / np, p = q - r * np;
-    const cx<R> zk = B[k * NB + r * 4 + p];
-    const cx<R> zm = cconj(B[((H - k) % H) * NB + r * 4 + p]);
-    const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
-    cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
-    x = cscale(x, sc);
-    if (k == 0) x.y = 0;
-    P[((size_t)p * MZ + (z0 + r)) * Kx + k] = x;
-  }
-}
-
-// ------------------------------------------------------------------------------------------------
-template <typename R, int MZ, int T, int NTH>
-__global__ void __launch_bounds__(NTH) zfwd_kernel(PlaneArgs a) {
-  __shared__ cx<R> s[MZ * T];
-  constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
-  const int w = a.w0 + blockIdx.y;
-  const int i0 = blockIdx.x * T;
-  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
-  const bool vpl = w >= a.nyu;
-  const int j = vpl ? w - a.nyu + 1 : w + 1;
-  const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
-  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)(w - a.w0) * 4 * MZ * Kx;
-  const double inv = 1.0 / (double)MZ;
-  double2 acc0[PERMAX], acc1[PERMAX], acc2[PERMAX];
-  const int np = vpl ? 2 : 4;
-  for (int p = 0; p < np; p++) {
-    for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
-      const int z = idx / T, t = idx - z * T, i = i0 + t;
-      s[idx] = i < Kx ? P[((size_t)p * MZ + z) * Kx + i] : mk<R>(0, 0);
-    }
-    __syncthreads();
-    fft_lds<R, MZ, T, NTH, -1>(s, tw);
-#pragma unroll
-    for (int q = 0; q < PERMAX; q++) {
-      const int idx = threadIdx.x + q * NTH;
-      if (idx < Nz * T) {
-        const int k = idx / T, t = idx - k * T, i = i0 + t;
-        const int zp = k < Nz / 2 ? k : k + MZ - Nz;
-        double2 v = to_d2(s[zp * T + t]);
-        v = d2(inv * v.x, inv * v.y);
-        const double kx = a.alp * (double)i;
-        const double kz = k < Nz / 2 ? a.bet * (double)k : -(a.bet * (double)(Nz - k));
-        if (!vpl) {
-          if (p == 0) acc0[q] = imul(kx, v);                    // der_x(uu)
-          else if (p == 1) { acc0[q] = cadd(acc0[q], imul(kz, v)); acc1[q] = imul(kx, v); }  // der_z(uw), der_x(uw)
-          else if (p == 2) acc2[q] = v;                         // vv (flux, unmasked)
-          else acc1[q] = cadd(acc1[q], imul(kz, v));            // der_z(ww)
-        } else {
-          if (p == 0) { acc0[q] = imul(kx, v); acc1[q] = v; }   // der_x(uv), uv flux
-          else { acc0[q] = cadd(acc0[q], imul(kz, v)); acc2[q] = v; }  // der_z(wv), wv flux
-        }
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int q = 0; q < PERMAX; q++) {
-    const int idx = threadIdx.x + q * NTH;
-    if (idx < Nz * T) {
-      const int k = idx / T, t = idx - k * T, i = i0 + t;
-      if (i < Kx) {
-        const size_t col = (size_t)k * Kx + i;
-        const bool masked = (k == Nz / 2);  // kz = -Nz/2 row is inside der_x/der_z's zero band
-        if (!vpl) {
-          a.Nxz1[(size_t)(j - 1) * C + col] = masked ? d2(0.0, 0.0) : acc0[q];
-          a.Nxz3[(size_t)(j - 1) * C + col] = masked ? d2(0.0, 0.0) : acc1[q];
-          a.vvc[(size_t)j * C + col] = acc2[q];
-        } else {
-          if (j <= a.nyv) a.Nxz2[(size_t)(j - 1) * C + col] = masked ? d2(0.0, 0.0) : acc0[q];
-          a.uvf[(size_t)j * C + col] = acc1[q];
-          a.wvf[(size_t)j * C + col] = acc2[q];
-        }
-      }
-    }
-  }
-}
 
 // ghost plane 0 of u1/u3 is still spectral when dtc_calc reads it (FOU3D.f90:622-626 against the
 // allocation at start.f90:549-551): its |Re|, |Im| join the CFL maxima.
@@ -241,8 +17,24 @@ __global__ void cfl_ghost_kernel(const double2 *u1, const double2 *u3, int C, do
   m1 = block_max<256>(m1, red);
   m3 = block_max<256>(m3, red);
   if (threadIdx.x == 0) {
-    atomic_max_nonneg(&cfl[0], m1);
-    atomic_max_nonneg(&cfl[1], m3);
+    const int slot = blockIdx.x & (CFL_SLOTS - 1);
+    atomic_max_nonneg(&cfl[slot], m1);
+    atomic_max_nonneg(&cfl[CFL_SLOTS + slot], m3);
+  }
+}
+
+// max over the slots -> out[0] = max|u1|, out[1] = max|u3| (one block of CFL_SLOTS threads)
+__global__ void cfl_reduce_kernel(const double *slots, double *out) {
+  const int l = threadIdx.x;
+  double m1 = slots[l], m3 = slots[CFL_SLOTS + l];
+  static_assert(CFL_SLOTS == 64, "one wavefront");
+  for (int off = 32; off > 0; off >>= 1) {
+    m1 = fmax(m1, __shfl_xor(m1, off, 64));
+    m3 = fmax(m3, __shfl_xor(m3, off, 64));
+  }
+  if (l == 0) {
+    out[0] = m1;
+    out[1] = m3;
   }
 }
 
@@ -273,7 +65,7 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
       s[idx] = i < KX ? d2(pph[z * IG + 2 * i], pph[z * IG + 2 * i + 1]) : d2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<double, MZ, TZ, NTH, +1>(s, twz);
+    fft_lds<double, Lay<MZ, TZ>, NTH, +1>(s, twz);
     for (int idx = threadIdx.x; idx < MZ * TZ; idx += NTH) {
       const int z = idx / TZ, t = idx % TZ, i = i0 + t;
       if (i < KX) { pph[z * IG + 2 * i] = s[idx].x; pph[z * IG + 2 * i + 1] = s[idx].y; }
@@ -293,7 +85,7 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
       s[n * 8 + r] = cadd(sum, cmul_si<1>(cmul(dif, cconj(twx[n]))));
     }
     __syncthreads();
-    fft_lds<double, H, 8, NTH, +1>(s, twh);
+    fft_lds<double, Lay<H, 8>, NTH, +1>(s, twh);
     for (int idx = threadIdx.x; idx < H * 8; idx += NTH) {
       const int r = idx / H, n = idx % H, z = z0 + r;
       if (z < MZ) { pph[z * IG + 2 * n] = s[n * 8 + r].x; pph[z * IG + 2 * n + 1] = s[n * 8 + r].y; }
@@ -309,7 +101,7 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
       s[n * 8 + r] = z < MZ ? d2(pbk[z * IG + 2 * n], pbk[z * IG + 2 * n + 1]) : d2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<double, H, 8, NTH, -1>(s, twh);
+    fft_lds<double, Lay<H, 8>, NTH, -1>(s, twh);
     for (int idx = threadIdx.x; idx < (H + 1) * 8; idx += NTH) {
       const int r = idx / (H + 1), k = idx % (H + 1), z = z0 + r;
       if (z < MZ) {
@@ -332,7 +124,7 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
       s[idx] = i < KX ? d2(pbk[z * IG + 2 * i], pbk[z * IG + 2 * i + 1]) : d2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<double, MZ, TZ, NTH, -1>(s, twz);
+    fft_lds<double, Lay<MZ, TZ>, NTH, -1>(s, twz);
     for (int idx = threadIdx.x; idx < MZ * TZ; idx += NTH) {
       const int z = idx / TZ, t = idx % TZ, i = i0 + t;
       if (i < KX) {
@@ -346,32 +138,35 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
 }
 
 // ------------------------------------------------------------------------------------------------
-// launchers
-template <int MX> constexpr int xpass_rows() { return MX >= 1152 ? 1 : (MX >= 576 ? 2 : 4); }
-
-int xpass_rows_per_block(int Mx) { return Mx >= 1152 ? 1 : (Mx >= 576 ? 2 : 4); }
-
-template <typename R, int MZ>
-static void launch_zinv_t(const PlaneArgs &a, int nw, hipStream_t st) {
-  constexpr int T = 8, NTH = 256;
-  dim3 g((a.Kx + T - 1) / T, nw);
-  zinv_kernel<R, MZ, T, NTH><<<g, NTH, 0, st>>>(a);
-}
-template <typename R, int MZ>
-static void launch_zfwd_t(const PlaneArgs &a, int nw, hipStream_t st) {
-  constexpr int T = 8, NTH = 256;
-  dim3 g((a.Kx + T - 1) / T, nw);
-  zfwd_kernel<R, MZ, T, NTH><<<g, NTH, 0, st>>>(a);
-}
-template <typename R, int MX>
-static void launch_xpass_t(const PlaneArgs &a, int nw, hipStream_t st) {
-  constexpr int RB = xpass_rows<MX>();
-  constexpr int NTH = 256;
-  dim3 g(a.Mz / RB, nw);
-  xpass_kernel<R, MX, RB, NTH><<<g, NTH, 0, st>>>(a);
-}
-
+// launchers.  The radix plan is fixed per transform length (so every tile variant of a kernel gives the
+// same bits); the tile variants differ in columns/rows per workgroup and threads, and the autotuner
+// picks among them.  Plans and tiles of the tuned lengths were chosen by timing on MI355X:
+//   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column);
+//   x-pass half-length 288: 16.18 (two passes).
 #define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
+#define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
+
+template <int N> struct ZPlanT { using type = GreedyPlan<N>; };
+template <> struct ZPlanT<576> { using type = Plan<12, 12, 4>; };
+template <> struct ZPlanT<1152> { using type = Plan<12, 12, 8>; };
+template <int H> struct XPlanT { using type = GreedyPlan<H>; };
+template <> struct XPlanT<288> { using type = Plan<16, 18>; };
+template <> struct XPlanT<576> { using type = Plan<24, 24>; };
+
+constexpr size_t LDS_MAX = 160 * 1024;
+
+// z-kernel tiles (T columns, NTH threads); 12.12.x plans: 48 threads per column (one butterfly each),
+// or 24 (two butterflies each).
+template <int N, int V> struct ZTile {
+  static constexpr bool p12 = (N == 576 || N == 1152);
+  static constexpr int T = p12 ? (V == 1 ? 4 : 8) : (V == 0 ? 8 : 4);
+  static constexpr int NTH = p12 ? (V == 2 ? 24 * T : 48 * T) : (V == 2 ? 128 : 256);
+};
+// x-pass tiles (RB rows, NTH threads)
+template <int MX, int V> struct XTile {
+  static constexpr int RB = V == 0 ? 4 : V == 1 ? 2 : 8;
+  static constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 512;
+};
 
 bool plane_size_supported(int n) {
 #define CASE(N) if (n == N) return true;
@@ -380,51 +175,101 @@ bool plane_size_supported(int n) {
   return false;
 }
 
-template <typename R>
-static int launch_plane_stage_t(const PlaneArgs &a, int nw, hipStream_t st) {
-  bool ok = false;
-#define CASEZ(N) if (a.Mz == N) { launch_zinv_t<R, N>(a, nw, st); ok = true; }
-  DNS_SIZES(CASEZ)
-#undef CASEZ
-  if (!ok) return -1;
-  ok = false;
-#define CASEX(N) if (a.Mx == N) { launch_xpass_t<R, N>(a, nw, st); ok = true; }
-  DNS_SIZES(CASEX)
-#undef CASEX
-  if (!ok) return -1;
-  ok = false;
-#define CASEZ2(N) if (a.Mz == N) { launch_zfwd_t<R, N>(a, nw, st); ok = true; }
-  DNS_SIZES(CASEZ2)
-#undef CASEZ2
-  return ok ? 0 : -1;
-}
+int xpass_rows_per_block(int Mx) { (void)Mx; return 4; }  // rows of the default x-pass tile
 
-int launch_plane_stage(const PlaneArgs &a, int nw, bool fp32, hipStream_t st) {
-  return fp32 ? launch_plane_stage_t<float>(a, nw, st) : launch_plane_stage_t<double>(a, nw, st);
-}
+int plane_variants(int n) { return DNS_TUNED(n) ? 3 : 1; }
 
-int launch_plane_stage_split(const PlaneArgs &a, int nw, bool fp32, hipStream_t st, int which) {
-  bool ok = false;
-  if (which == 0) {
-#define CZ(N) if (a.Mz == N) { if (fp32) launch_zinv_t<float, N>(a, nw, st); else launch_zinv_t<double, N>(a, nw, st); ok = true; }
-    DNS_SIZES(CZ)
-#undef CZ
-  } else if (which == 1) {
-#define CXP(N) if (a.Mx == N) { if (fp32) launch_xpass_t<float, N>(a, nw, st); else launch_xpass_t<double, N>(a, nw, st); ok = true; }
-    DNS_SIZES(CXP)
-#undef CXP
+template <typename R, int MZ, int V>
+static void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
+  using Z = ZTile<MZ, V>;
+  if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
+    zinv_v<R, MZ, 0>(a, nw, st);
   } else {
-#define CZF(N) if (a.Mz == N) { if (fp32) launch_zfwd_t<float, N>(a, nw, st); else launch_zfwd_t<double, N>(a, nw, st); ok = true; }
-    DNS_SIZES(CZF)
-#undef CZF
+    dim3 g((a.Kx + Z::T - 1) / Z::T, nw, 3);
+    zinv_kernel<R, MZ, Z::T, Z::NTH, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
   }
-  return ok ? 0 : -1;
+}
+template <typename R, int MZ, int V, bool VPL>
+static void zfwd_v(const PlaneArgs &a, int nw, hipStream_t st) {
+  using Z = ZTile<MZ, V>;
+  if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
+    zfwd_v<R, MZ, 0, VPL>(a, nw, st);
+  } else {
+    dim3 g((a.Kx + Z::T - 1) / Z::T, nw);
+    zfwd_kernel<R, MZ, Z::T, Z::NTH, VPL, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
+  }
+}
+template <typename R, int MX, int V, bool VPL>
+static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
+  using X = XTile<MX, V>;
+  constexpr int H = MX / 2;
+  if constexpr (V > 0 && (size_t)X::RB * 4 * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
+    xpass_v<R, MX, 0, VPL>(a, nw, st);
+  } else {
+    if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // default tile: 4 rows divide Mz
+    dim3 g(a.Mz / X::RB, nw);
+    using PL = typename XPlanT<H>::type;
+    xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);
+  }
+}
+
+template <typename R, int N>
+static void zinv_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
+  if constexpr (DNS_TUNED(N)) {
+    if (v == 1) return zinv_v<R, N, 1>(a, nw, st);
+    if (v == 2) return zinv_v<R, N, 2>(a, nw, st);
+  }
+  zinv_v<R, N, 0>(a, nw, st);
+}
+template <typename R, int N, bool VPL>
+static void zfwd_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
+  if constexpr (DNS_TUNED(N)) {
+    if (v == 1) return zfwd_v<R, N, 1, VPL>(a, nw, st);
+    if (v == 2) return zfwd_v<R, N, 2, VPL>(a, nw, st);
+  }
+  zfwd_v<R, N, 0, VPL>(a, nw, st);
+}
+template <typename R, int N, bool VPL>
+static void xpass_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
+  if constexpr (DNS_TUNED(N)) {
+    if (v == 1) return xpass_v<R, N, 1, VPL>(a, nw, st);
+    if (v == 2) return xpass_v<R, N, 2, VPL>(a, nw, st);
+  }
+  xpass_v<R, N, 0, VPL>(a, nw, st);
+}
+
+// which: 0 z-inverse, 1 x-pass u-planes, 2 x-pass v-planes, 3 z-forward u-planes, 4 z-forward v-planes.
+// The caller passes planes [a.w0, a.w0 + nw) of the right type for which = 1..4.
+template <typename R>
+static int launch_t(const PlaneArgs &a, int nw, int which, int v, hipStream_t st) {
+  const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
+#define CASE(N)                                                       \
+  if (n == N) {                                                       \
+    switch (which) {                                                  \
+      case 0: zinv_sized<R, N>(a, nw, v, st); return 0;               \
+      case 1: xpass_sized<R, N, false>(a, nw, v, st); return 0;       \
+      case 2: xpass_sized<R, N, true>(a, nw, v, st); return 0;        \
+      case 3: zfwd_sized<R, N, false>(a, nw, v, st); return 0;        \
+      case 4: zfwd_sized<R, N, true>(a, nw, v, st); return 0;         \
+    }                                                                 \
+  }
+  DNS_SIZES(CASE)
+#undef CASE
+  return -1;
+}
+
+int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st) {
+  return fp32 ? launch_t<float>(a, nw, which, variant, st) : launch_t<double>(a, nw, which, variant, st);
 }
 
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st) {
   int nb = (C + 255) / 256;
   if (nb > 1024) nb = 1024;
   cfl_ghost_kernel<<<nb, 256, 0, st>>>(u1, u3, C, cfl);
+}
+
+void launch_cfl_reduce(const double *slots, double *out, hipStream_t st) {
+  cfl_reduce_kernel<<<1, CFL_SLOTS, 0, st>>>(slots, out);
 }
 
 void launch_zero(double *p, int n, hipStream_t st) { zero_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n); }

@@ -42,7 +42,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_set_dt", "dns_rhs0", "dns_nonlinear", "dns_solveU", "dns_meanflow_ctU", "dns_solveP",
            "dns_vcorr", "dns_meanflow_ctP", "dns_rk3_substep", "dns_rk3_step", "dns_rk3_step_graph",
            "dns_plane_fft_roundtrip", "dns_modes_to_planes", "dns_planes_to_modes", "dns_sync",
-           "dns_set_timing", "dns_get_timing"]
+           "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
+           "dns_autotune"]
 
 
 def lib():
@@ -76,6 +77,10 @@ def lib():
     L.dns_modes_to_planes.argtypes = [vp, vp, ctypes.c_int, vp]
     L.dns_planes_to_modes.argtypes = [vp, vp, ctypes.c_int, vp]
     L.dns_get_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.dns_get_plane_variants.argtypes = [vp, _ip, _ip]
+    L.dns_set_plane_variants.argtypes = [vp, _ip]
+    L.dns_autotune.argtypes = [vp, ctypes.c_int, _ip, _dp]
     _lib = L
     return L
 
@@ -208,6 +213,29 @@ class DNS:
         _chk(lib().dns_get_timing(self.h, _p(t), ctypes.byref(n)))
         return dict(zinv=t[0], xpass=t[1], zfwd=t[2], col_up=t[3], col_down=t[4], col_final=t[5],
                     substep=t[6], n=n.value)
+
+    # ---- plane-kernel tile variants and the autotuner
+    KERNELS = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v")
+
+    def plane_variants(self):
+        """(current variant per kernel, number of variants per kernel), both int arrays of 5."""
+        v = np.zeros(5, np.int32)
+        c = np.zeros(5, np.int32)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        _chk(lib().dns_get_plane_variants(self.h, v.ctypes.data_as(_ip), c.ctypes.data_as(_ip)))
+        return v, c
+
+    def set_plane_variants(self, var):
+        v = np.ascontiguousarray(var, np.int32)
+        assert v.shape == (5,)
+        _chk(lib().dns_set_plane_variants(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def autotune(self, iters=5):
+        """Times every variant; returns (chosen[5], ms[5, 3]) with -1 for missing variants."""
+        ch = np.zeros(5, np.int32)
+        ms = np.zeros(15)
+        _chk(lib().dns_autotune(self.h, iters, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms)))
+        return ch, ms.reshape(5, 3)
 
     # ---- device-pointer utilities (torch tensors provide the device memory)
     def plane_fft_roundtrip(self, d_in, d_phys, d_back, n):

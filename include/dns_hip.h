@@ -130,6 +130,21 @@ int dns_sync(dns_handle *h);
 int dns_set_timing(dns_handle *h, int on);
 int dns_get_timing(dns_handle *h, double *t, int *n);
 
+/* Tile variants of the five plane kernels, index c: 0 z-inverse, 1 x-pass of u-planes, 2 x-pass of
+ * v-planes, 3 z-forward of u-planes, 4 z-forward of v-planes.  All variants of a kernel give
+ * bit-identical results.  No counterpart in the reference (FFTPACK has one code path); this replaces
+ * the reference's fixed choice of loop order in four_to_phys_du / phys_to_four_du (FOU3D.f90:857-1231).
+ * dns_get_plane_variants: var[5] current choice, count[5] (may be NULL) variants available.
+ * dns_set_plane_variants: var[5]; status 1 for a variant that does not exist. */
+int dns_get_plane_variants(dns_handle *h, int *var, int *count);
+int dns_set_plane_variants(dns_handle *h, const int *var);
+/* Autotuner: times each variant of each plane kernel (iters launches over the first plane batch) and
+ * keeps the fastest.  chosen[5] (may be NULL) receives the choice, ms[15] (may be NULL) the time per
+ * launch of variant v of kernel c at ms[3c+v] (-1 where the variant does not exist).  Rewrites the
+ * plane-stage scratch (nonlinear terms and fluxes of the current substep): call it before a step or
+ * between steps.  Drops a captured graph (the next dns_rk3_step_graph captures the new choice). */
+int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

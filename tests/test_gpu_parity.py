@@ -56,6 +56,8 @@ def test_fused_steps_vs_reference_golden(golden, ctp):
     (48, 48, 153, 10, 1),     # 32 x 32 modes x 153: the reference's CPU-runnable case
     (96, 96, 65, 5, 2),
     (192, 192, 153, 3, 3),    # 128 x 128 x 153
+    (576, 576, 17, 2, 5),     # the 384 x 384 plane transforms (tuned radix plans), few planes
+    (1152, 576, 9, 1, 6),     # 768 x 384 plane transforms
 ])
 def test_fused_steps_vs_oracle(Mx, Mz, ny, nsteps, seed):
     print("case: fused %dx%dx%d %d steps" % (Mx, Mz, ny, nsteps), flush=True)
@@ -253,3 +255,52 @@ def test_full_size_properties_384():
                       precision=dnship.PREC_MIXED)
     for x, y in zip(a[:3], b[:3]):
         assert rel_l2(y, x) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
+@pytest.mark.parametrize("Mx,Mz,ny", [(192, 192, 33), (576, 576, 17)])
+def test_plane_tile_variants_bit_identical(prec, Mx, Mz, ny):
+    """Every tile variant of every plane kernel gives the same bits (tuned lengths have three)."""
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=4, broadband=True)
+    ref = None
+    for var in ([0, 0, 0, 0, 0], [1, 1, 1, 1, 1], [2, 2, 2, 2, 2], [1, 2, 0, 2, 1]):
+        g = dnship.DNS(Mx, Mz, ny, precision=prec, plane_batch=11)
+        cur, cnt = g.plane_variants()
+        assert list(cur) == [0] * 5 and list(cnt) == [3] * 5
+        print("case: variants %s prec %d" % (var, prec), flush=True)
+        g.set_plane_variants(var)
+        g.upload(*st)
+        g.step(2)
+        g.sync()
+        out = g.download() + g.download_nonlinear()
+        g.close()
+        if ref is None:
+            ref = out
+        else:
+            for x, y in zip(ref, out):
+                assert np.array_equal(x, y)
+
+
+def test_autotune_choice_valid_and_results_unchanged():
+    Mx, Mz, ny = 192, 192, 33
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=5)
+    a, _, _ = gpu_run(Mx, Mz, ny, st, 2)
+    g = dnship.DNS(Mx, Mz, ny)
+    g.upload(*st)
+    ch, ms = g.autotune(iters=2)
+    assert np.all((ch >= 0) & (ch < 3))
+    assert np.all(ms > 0)
+    assert list(g.plane_variants()[0]) == list(ch)
+    g.upload(*st)           # autotuning rewrote plane-stage scratch only; start from the same state
+    g.step_graph(2)
+    g.sync()
+    b = g.download()
+    g.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    h = dnship.DNS(48, 48, 33)
+    _, cnt = h.plane_variants()
+    assert list(cnt) == [1] * 5
+    with pytest.raises(dnship.DnsError):
+        h.set_plane_variants([0, 1, 0, 0, 0])
+    h.close()
