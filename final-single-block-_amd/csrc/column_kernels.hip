@@ -433,6 +433,128 @@ __global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a) {
   a.w3u[(size_t)(nyu + 1) * C + col] = 0.0 * w2;
 }
 
+// The same sweep with the three solves on three wavefronts of one workgroup (wave 0: u1, wave 1: u3,
+// wave 2: u2; 64 columns per workgroup): three times the wavefronts of the thread-per-column form, and
+// each one streams a third of the rows, so more memory requests are in flight.  With PF the next row's
+// inputs are loaded before the current row is computed.  Arithmetic identical to fused_up_kernel.
+template <bool PF>
+__global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
+  const int f = threadIdx.x >> 6;
+  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (col >= a.C) return;
+  const int C = a.C, nyu = a.nyu, nyv = a.nyv;
+  const Mode m = col_mode(a, col);
+  const double dt = substep_dt(a, f == 0 ? col : -1);  // the state's dt and t are written once
+  const double C1 = a.aRK / a.Re, C2 = -a.cRK, Cg = -a.gRK;
+  const double beta = a.bRK * dt / a.Re;
+  const double k2 = m.k2x + m.k2z;
+  double w2 = 1.0 / 1.0;
+  if (f < 2) {
+    double2 *U = f == 0 ? a.u1 : a.u3;
+    double2 *Nu = f == 0 ? a.Nu1 : a.Nu3;
+    const double2 *Nxz = f == 0 ? a.Nxz1 : a.Nxz3;
+    const double2 *fl = f == 0 ? a.uvf : a.wvf;
+    const double c3 = f == 0 ? -(a.dRK * m.kx) : -(a.dRK * m.kz);
+    const double Cm = col == 0 ? -(a.dRK * a.st->mpgx) : 0.0;
+    double2 um = U[col], uc = U[(size_t)C + col];
+    double2 flm = fl[col];
+    double2 pc = a.p[col];
+    double2 y = rmul(w2, d2(0.0, 0.0));
+    double w3 = a.gw1 * w2;
+    U[col] = y;
+    if (f == 0) a.w3u[col] = w3;
+    struct Row { double2 up, flc, nu, nx, pn; };
+    auto load = [&](int j) {
+      Row r;
+      r.up = U[(size_t)(j + 1) * C + col];
+      r.flc = fl[(size_t)j * C + col];
+      r.nu = Nu[(size_t)(j - 1) * C + col];
+      r.nx = Nxz[(size_t)(j - 1) * C + col];
+      r.pn = (j <= nyv) ? a.p[(size_t)j * C + col] : d2(0.0, 0.0);
+      return r;
+    };
+    Row cur = load(1);
+    for (int j = 1; j <= nyu; j++) {
+      Row nxt;
+      if (PF && j < nyu) nxt = load(j + 1);
+      const double d1 = D2U(1, j), dd2 = k2 + D2U(2, j), d3 = D2U(3, j);
+      const double2 L = cadd(cadd(rmul(d1, um), rmul(dd2, uc)), rmul(d3, cur.up));
+      double2 r = cadd(cadd(rmul(C1, L), rmul(C2, cur.nu)), imul(c3, pc));
+      if (f == 0) r.x = r.x + Cm;
+      const double2 N = cadd(cur.nx, rmul(a.dthdyu[j], csub(cur.flc, flm)));
+      r = cadd(r, rmul(Cg, N));
+      Nu[(size_t)(j - 1) * C + col] = N;
+      const double2 rhs = d2(uc.x + dt * r.x, uc.y + dt * r.y);
+      const double a1 = -beta * D2U(1, j), a2 = 1.0 - beta * (D2U(2, j) + m.k2x + m.k2z), a3 = -beta * D2U(3, j);
+      w2 = 1.0 / (a2 - a1 * w3);
+      y = d2((rhs.x - a1 * y.x) * w2, (rhs.y - a1 * y.y) * w2);
+      w3 = a3 * w2;
+      U[(size_t)j * C + col] = y;
+      if (f == 0) a.w3u[(size_t)j * C + col] = w3;
+      um = uc;
+      uc = cur.up;
+      flm = cur.flc;
+      if (j <= nyv) pc = cur.pn;
+      if (!PF && j < nyu) nxt = load(j + 1);
+      cur = nxt;
+    }
+    w2 = 1.0 / (1.0 - a.gw2 * w3);
+    y = d2((0.0 - a.gw2 * y.x) * w2, (0.0 - a.gw2 * y.y) * w2);
+    U[(size_t)(nyu + 1) * C + col] = y;
+    if (f == 0) a.w3u[(size_t)(nyu + 1) * C + col] = 0.0 * w2;
+  } else {
+    double2 u2m = a.u2[col], u2c = a.u2[(size_t)C + col];
+    double2 vvc_c = a.vvc[(size_t)C + col];
+    double2 pc = a.p[col];
+    double2 y2 = rmul(w2, d2(0.0, 0.0));
+    double w3v = 0.0 * w2;
+    a.u2[col] = y2;
+    a.w3v[col] = w3v;
+    struct Row { double2 u2p, pn, vvn, nu, nx; };
+    auto load = [&](int j) {
+      Row r;
+      const bool in = j <= nyv;
+      r.u2p = in ? a.u2[(size_t)(j + 1) * C + col] : d2(0.0, 0.0);
+      r.pn = in ? a.p[(size_t)j * C + col] : d2(0.0, 0.0);
+      r.vvn = in ? a.vvc[(size_t)(j + 1) * C + col] : d2(0.0, 0.0);
+      r.nu = in ? a.Nu2[(size_t)(j - 1) * C + col] : d2(0.0, 0.0);
+      r.nx = in ? a.Nxz2[(size_t)(j - 1) * C + col] : d2(0.0, 0.0);
+      return r;
+    };
+    Row cur = load(1);
+    for (int j = 1; j <= nyu; j++) {
+      Row nxt;
+      if (PF && j < nyu) nxt = load(j + 1);
+      if (j <= nyv) {
+        const double e1 = D2V(1, j), e2 = k2 + D2V(2, j), e3 = D2V(3, j);
+        const double2 L2 = cadd(cadd(rmul(e1, u2m), rmul(e2, u2c)), rmul(e3, cur.u2p));
+        const double C3 = -((a.dRK * 1.0) * a.dthdyv[j]);
+        double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, cur.nu)), rmul(C3, csub(cur.pn, pc)));
+        const double2 N2 = cadd(cur.nx, rmul(a.dthdyv[j], csub(cur.vvn, vvc_c)));
+        r2 = cadd(r2, rmul(Cg, N2));
+        a.Nu2[(size_t)(j - 1) * C + col] = N2;
+        const double2 rhs2 = d2(u2c.x + dt * r2.x, u2c.y + dt * r2.y);
+        const double a1 = -beta * D2V(1, j), a2 = 1.0 - beta * (D2V(2, j) + m.k2x + m.k2z), a3 = -beta * D2V(3, j);
+        w2 = 1.0 / (a2 - a1 * w3v);
+        y2 = d2((rhs2.x - a1 * y2.x) * w2, (rhs2.y - a1 * y2.y) * w2);
+        w3v = a3 * w2;
+        pc = cur.pn;
+        vvc_c = cur.vvn;
+      } else {  // top wall row of v: (0, 1, 0), rhs 0
+        w2 = 1.0 / (1.0 - 0.0 * w3v);
+        y2 = d2((0.0 - 0.0 * y2.x) * w2, (0.0 - 0.0 * y2.y) * w2);
+        w3v = 0.0 * w2;
+      }
+      a.u2[(size_t)j * C + col] = y2;
+      a.w3v[(size_t)j * C + col] = w3v;
+      u2m = u2c;
+      u2c = cur.u2p;
+      if (!PF && j < nyu) nxt = load(j + 1);
+      cur = nxt;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) fused_down_kernel(ColArgs a) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
@@ -591,7 +713,11 @@ void launch_meanflow(const ColArgs &a, hipStream_t st) { meanflow_kernel<<<1, 64
 void launch_qx(const ColArgs &a, hipStream_t st) { qx_kernel<<<1, 64, 0, st>>>(a); }
 void launch_solveP(const ColArgs &a, hipStream_t st) { solveP_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
 void launch_vcorr(const ColArgs &a, hipStream_t st) { vcorr_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_fused_up(const ColArgs &a, hipStream_t st) { fused_up_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
+void launch_fused_up(const ColArgs &a, int variant, hipStream_t st) {
+  if (variant == 1) fused_up3_kernel<false><<<nblk(a.C, 64), 192, 0, st>>>(a);
+  else if (variant == 2) fused_up3_kernel<true><<<nblk(a.C, 64), 192, 0, st>>>(a);
+  else fused_up_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a);
+}
 void launch_fused_down(const ColArgs &a, hipStream_t st) { fused_down_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
 void launch_fused_final(const ColArgs &a, hipStream_t st) {
   fused_final_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a);

@@ -40,8 +40,9 @@ struct dns_handle {
   dns_params prm;
   int Mx, Mz, Nx, Nz, Kx, C, ny, nyu, nyv;
   bool fp32;
-  int batch;  // planes per plane-stage group
-  int var[5] = {0, 0, 0, 0, 0};  // tile variant per plane kernel (z-inverse, x-pass u/v, z-forward u/v)
+  int bj;  // plane index j per plane-stage group (u- and v-planes of each j)
+  // variant per kernel: z-inverse, x-pass u/v, z-forward u/v, column sweeps up/down/final
+  int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0};
   size_t bytes = 0;
   hipStream_t stream = nullptr;
   bool uploaded = false, timing = false;
@@ -223,7 +224,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   h->nyv = ny - 2; h->nyu = h->nyv + 1;
   h->fp32 = prm->precision == DNS_PREC_MIXED;
   const int nplanes = 2 * h->nyu;
-  h->batch = (prm->plane_batch > 0 && prm->plane_batch < nplanes) ? prm->plane_batch : nplanes;
+  h->bj = (prm->plane_batch > 0 && prm->plane_batch < nplanes) ? (prm->plane_batch + 1) / 2 : h->nyu;
   host_grid(h);
 
   auto chk = [&](int rc) { return rc; };
@@ -244,7 +245,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   rc |= chk(dalloc(h, &h->stage, C * ru));
   rc |= chk(dalloc(h, &h->st, 1));
   const size_t rsz = h->fp32 ? sizeof(float2) : sizeof(double2);
-  const size_t iel = (size_t)h->batch * 3 * Mz * h->Kx, pel = (size_t)h->batch * 4 * Mz * h->Kx;
+  const size_t iel = (size_t)2 * h->bj * 3 * Mz * h->Kx, pel = (size_t)2 * h->bj * 4 * Mz * h->Kx;
   char *Ib = nullptr, *Pb = nullptr, *tz = nullptr, *tx = nullptr, *th = nullptr;
   rc |= chk(dalloc(h, &Ib, iel * rsz)); rc |= chk(dalloc(h, &Pb, pel * rsz));
   rc |= chk(dalloc(h, &tz, (size_t)Mz * rsz)); rc |= chk(dalloc(h, &tx, (size_t)Mx * rsz));
@@ -401,20 +402,17 @@ static int plane_stage(dns_handle *h, int kRK) {
     dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, h->cfls, h->stream);
   }
   PlaneArgs a = plane_args(h, kRK);
-  const int total = 2 * h->nyu;
-  const bool one_group = h->batch >= total;  // per-kernel event marks only with one launch per kernel
+  const bool one_group = h->bj >= h->nyu;  // per-kernel event marks only with one launch per kernel
   tmark(h, 0);
-  for (int b0 = 0; b0 < total; b0 += h->batch) {
-    const int b1 = std::min(b0 + h->batch, total);
-    const int um = std::min(b1, h->nyu), vm = std::max(b0, h->nyu);  // u-planes [b0, um), v-planes [vm, b1)
-    a.wbase = b0;
-    int rc = 0;
-    a.w0 = b0;
-    rc |= dnsk::launch_plane_kernel(a, b1 - b0, h->fp32, 0, h->var[0], h->stream);
+  for (int j0 = 1; j0 <= h->nyu; j0 += h->bj) {
+    const int nb = std::min(h->bj, h->nyu - j0 + 1);
+    a.j0 = j0;
+    a.nbj = nb;
+    int rc = dnsk::launch_plane_kernel(a, nb + 1, h->fp32, 0, h->var[0], h->stream);
     if (one_group) tmark(h, 1);
-    for (int k = 0; k < 2; k++) {  // x-pass, then z-forward
-      if (um > b0) { a.w0 = b0; rc |= dnsk::launch_plane_kernel(a, um - b0, h->fp32, 1 + 2 * k, h->var[1 + 2 * k], h->stream); }
-      if (b1 > vm) { a.w0 = vm; rc |= dnsk::launch_plane_kernel(a, b1 - vm, h->fp32, 2 + 2 * k, h->var[2 + 2 * k], h->stream); }
+    for (int k = 0; k < 2; k++) {  // x-pass, then z-forward; u-planes, then v-planes
+      rc |= dnsk::launch_plane_kernel(a, nb, h->fp32, 1 + 2 * k, h->var[1 + 2 * k], h->stream);
+      rc |= dnsk::launch_plane_kernel(a, nb, h->fp32, 2 + 2 * k, h->var[2 + 2 * k], h->stream);
       if (one_group) tmark(h, 2 + k);
     }
     if (rc) return fail(DNS_ERR_ARG, "plane stage: sizes %d x %d not instantiated", h->Mx, h->Mz);
@@ -617,7 +615,7 @@ static int fused_substep(dns_handle *h, int kRK) {
   int rc = plane_stage(h, kRK);
   if (rc) return rc;
   ColArgs a = col_args(h, kRK);
-  dnsk::launch_fused_up(a, h->stream);
+  dnsk::launch_fused_up(a, h->var[5], h->stream);
   tmark(h, 4);
   dnsk::launch_fused_down(a, h->stream);
   tmark(h, 5);
@@ -677,9 +675,14 @@ extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Plane-kernel tile variants.  Every variant of a kernel gives bit-identical results (same butterflies,
-// same order of operations per element); they differ in planes/rows per workgroup and threads.
-static int plane_len(const dns_handle *h, int which) { return (which == 1 || which == 2) ? h->Mx : h->Mz; }
+// Kernel variants.  Every variant of a kernel gives bit-identical results: the plane kernels' variants
+// differ in tile shape (same butterflies, same order of operations per element), the column sweep's in
+// how the three solves map to wavefronts (same arithmetic per column).
+static int kern_len(const dns_handle *h, int c) { return (c == 1 || c == 2) ? h->Mx : h->Mz; }
+static int kern_variants(const dns_handle *h, int c) {
+  if (c < 5) return dnsk::plane_variants(kern_len(h, c));
+  return c == 5 ? 3 : 1;
+}
 
 static void drop_graph(dns_handle *h) {
   if (h->graph) (void)hipGraphExecDestroy(h->graph);
@@ -688,53 +691,70 @@ static void drop_graph(dns_handle *h) {
 
 extern "C" int dns_get_plane_variants(dns_handle *h, int *var, int *count) {
   if (!h || !var) return fail(DNS_ERR_ARG, "dns_get_plane_variants: null argument");
-  for (int c = 0; c < 5; c++) {
+  for (int c = 0; c < DNS_NKERN; c++) {
     var[c] = h->var[c];
-    if (count) count[c] = dnsk::plane_variants(plane_len(h, c));
+    if (count) count[c] = kern_variants(h, c);
   }
   return DNS_OK;
 }
 
 extern "C" int dns_set_plane_variants(dns_handle *h, const int *var) {
   if (!h || !var) return fail(DNS_ERR_ARG, "dns_set_plane_variants: null argument");
-  for (int c = 0; c < 5; c++)
-    if (var[c] < 0 || var[c] >= dnsk::plane_variants(plane_len(h, c)))
-      return fail(DNS_ERR_ARG, "dns_set_plane_variants: kernel %d has no variant %d (length %d has %d)", c, var[c],
-                  plane_len(h, c), dnsk::plane_variants(plane_len(h, c)));
-  for (int c = 0; c < 5; c++) h->var[c] = var[c];
+  for (int c = 0; c < DNS_NKERN; c++)
+    if (var[c] < 0 || var[c] >= kern_variants(h, c))
+      return fail(DNS_ERR_ARG, "dns_set_plane_variants: kernel %d has no variant %d (it has %d)", c, var[c],
+                  kern_variants(h, c));
+  for (int c = 0; c < DNS_NKERN; c++) h->var[c] = var[c];
   drop_graph(h);
   return DNS_OK;
 }
 
-// Times every variant of every plane kernel on the first plane batch (kRK = 2 arguments: no CFL maxima)
-// and keeps the fastest.  The plane-stage outputs are scratch that the next substep rewrites, so this
-// may be called before a step or between steps, not between the staged calls of one substep.
+// Times every variant of every kernel that has more than one and keeps the fastest: plane kernels on
+// the first plane batch with kRK = 2 arguments (no CFL maxima), the column sweep on the whole field.
+// Rewrites scratch only (nonlinear terms, fluxes, eliminated rows), so it may be called before a step
+// or between steps, not between the staged calls of one substep; the fields are saved and restored.
 extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   if (!h) return fail(DNS_ERR_ARG, "dns_autotune: null handle");
   if (iters < 1 || iters > 1000) return fail(DNS_ERR_ARG, "dns_autotune: iters=%d not in 1..1000", iters);
   if (!h->uploaded) return fail(DNS_ERR_STATE, "dns_autotune: no fields uploaded");
   PlaneArgs a = plane_args(h, 2);
-  const int total = 2 * h->nyu, b1 = std::min(h->batch, total);
-  const int um = std::min(b1, h->nyu);
-  a.wbase = 0;
+  a.j0 = 1;
+  a.nbj = std::min(h->bj, h->nyu);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  // the column sweep overwrites u1, u2, u3 and Nu: keep copies and put them back afterwards
+  const size_t C = h->C;
+  const size_t nu = C * (h->nyu + 2), nv = C * (h->nyv + 2), nn = C * h->nyu;
+  double2 *save = nullptr;
   int rc = DNS_OK;
-  for (int c = 0; c < 5 && rc == DNS_OK; c++) {
-    int w0 = 0, nw = b1;  // z-inverse: the whole batch
-    if (c == 1 || c == 3) nw = um;
-    if (c == 2 || c == 4) { w0 = h->nyu; nw = b1 - h->nyu; }
-    const int nv = dnsk::plane_variants(plane_len(h, c));
+  if (kern_variants(h, 5) > 1) {
+    HIPCHK(hipMalloc(&save, (2 * nu + nv + 3 * nn) * sizeof(double2)));
+    double2 *q = save;
+    const double2 *src[6] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3};
+    const size_t len[6] = {nu, nu, nv, nn, nn, nn};
+    for (int k = 0; k < 6; k++) {
+      HIPCHK(hipMemcpyAsync(q, src[k], len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+      q += len[k];
+    }
+  }
+  ColArgs ca = col_args(h, 2);
+  for (int c = 0; c < DNS_NKERN && rc == DNS_OK; c++) {
+    const int nw = c == 0 ? a.nbj + 1 : a.nbj;  // the first plane batch
+    const int nv_ = kern_variants(h, c);
     int best = 0;
     double tbest = 1e300;
     for (int v = 0; v < 3; v++) {
       if (ms) ms[3 * c + v] = -1.0;
-      if (v >= nv || nw <= 0) continue;
-      a.w0 = w0;
-      if (dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream)) { rc = fail(DNS_ERR_ARG, "dns_autotune: size not instantiated"); break; }
+      if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0)) continue;
+      auto launch = [&]() -> int {
+        if (c < 5) return dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
+        dnsk::launch_fused_up(ca, v, h->stream);
+        return 0;
+      };
+      if (launch()) { rc = fail(DNS_ERR_ARG, "dns_autotune: size not instantiated"); break; }
       hipError_t e = hipEventRecord(e0, h->stream);
-      for (int it = 0; it < iters && e == hipSuccess; it++) dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
+      for (int it = 0; it < iters && e == hipSuccess; it++) launch();
       if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float t = 0.f;
@@ -746,6 +766,17 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
     }
     h->var[c] = best;
     if (chosen) chosen[c] = best;
+  }
+  if (save) {
+    double2 *q = save;
+    double2 *dst[6] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3};
+    const size_t len[6] = {nu, nu, nv, nn, nn, nn};
+    for (int k = 0; k < 6; k++) {
+      (void)hipMemcpyAsync(dst[k], q, len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream);
+      q += len[k];
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(save);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

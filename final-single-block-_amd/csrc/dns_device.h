@@ -248,7 +248,10 @@ __host__ __device__ constexpr int next_radix(int rem) {
 // take consecutive transforms (memory-coalesced when the transforms are adjacent columns);
 // BF = true: s[slot * LR + row(n)] and consecutive threads take consecutive butterflies (for transforms
 // that are contiguous rows).
-template <int N_, int NTR_, int NACT_ = NTR_, int GA_ = 1, int GS_ = 1, int PS_ = 0, bool BF_ = false>
+// TR = true maps active transform a to slot (a % GA) * GS + a / GA instead (transform-major order of the
+// active transforms, e.g. all rows of field 0 first).
+template <int N_, int NTR_, int NACT_ = NTR_, int GA_ = 1, int GS_ = 1, int PS_ = 0, bool BF_ = false,
+          bool TR_ = false>
 struct Lay {
   static constexpr int N = N_, NTR = NTR_, NACT = NACT_, PS = PS_;
   static constexpr bool BF = BF_;
@@ -256,13 +259,23 @@ struct Lay {
   static constexpr int SIZE = NTR * LR;                   // elements
   __device__ __forceinline__ static int row(int n) { return PS > 0 ? n + (n >> PS) : n; }
   __device__ __forceinline__ static int addr(int n, int t) { return BF ? t * LR + row(n) : row(n) * NTR + t; }
-  __device__ __forceinline__ static int slot(int a) { return (a / GA_) * GS_ + a % GA_; }
+  __device__ __forceinline__ static int slot(int a) {
+    return TR_ ? (a % GA_) * GS_ + a / GA_ : (a / GA_) * GS_ + a % GA_;
+  }
   // thread work item b of a pass with M butterflies per transform -> (active transform a, butterfly i)
   template <int M>
   __device__ __forceinline__ static void split(int b, int &a, int &i) {
     if (BF) { i = b % M; a = b / M; } else { a = b % NACT; i = b / NACT; }
   }
 };
+
+// Thread index, opaque to the optimiser: index arithmetic derived from it is recomputed where it is used
+// instead of being hoisted out of outer loops and kept live in registers across them.
+__device__ __forceinline__ int tidx() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
 
 // Butterfly of one pass on registers u[0..RAD): twiddle (P > 1), DFT.
 template <typename R, int N, int S, int P, int RAD>
@@ -287,7 +300,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   cx<R> u[PER][RAD];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = threadIdx.x + q * NTH;
+    const int b = tidx() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -299,7 +312,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = threadIdx.x + q * NTH;
+    const int b = tidx() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -369,7 +382,7 @@ __device__ __forceinline__ void fft_first(cx<R> *s, LD ld) {
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = threadIdx.x + q * NTH;
+    const int b = tidx() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -387,8 +400,10 @@ __device__ __forceinline__ void fft_first(cx<R> *s, LD ld) {
   __syncthreads();
 }
 
-// Last pass (radix RAD, span P = N/RAD) straight to memory: st(n, a, value) receives position n of
-// active transform a.  Only reads s: the caller needs a barrier before s is written again.
+// Last pass (radix RAD, span P = N/RAD) straight to memory: st(n, a, k, value) receives position n of
+// active transform a; k = q*RAD + r (q-th work item of this thread, r-th output) is a compile-time index
+// after unrolling, the same for the same (n, a) on every call, so a caller can keep per-output state in
+// registers across transforms.  Only reads s: the caller needs a barrier before s is written again.
 template <typename R, class L, int NTH, int S, int RAD, typename ST>
 __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict__ tw, ST st) {
   constexpr int P = L::N / RAD;
@@ -396,7 +411,7 @@ __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = threadIdx.x + q * NTH;
+    const int b = tidx() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<P>(b, a, i);
@@ -406,9 +421,16 @@ __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict
       for (int r = 0; r < RAD; r++) u[r] = s[L::addr(i + r * P, t)];
       butterfly<R, L::N, S, P, RAD>(u, i, tw);
 #pragma unroll
-      for (int r = 0; r < RAD; r++) st(i + r * P, a, u[r]);
+      for (int r = 0; r < RAD; r++) st(i + r * P, a, q * RAD + r, u[r]);
     }
   }
+}
+
+// Register slots a thread needs for fft_last's outputs (PER work items x RAD outputs).
+template <class L, int NTH, class PL>
+constexpr int last_slots() {
+  constexpr int RAD = plan_info<PL>::last, P = L::N / RAD;
+  return ((P * L::NACT + NTH - 1) / NTH) * RAD;
 }
 
 // Memory -> LDS -> memory: first pass from memory, middle passes in LDS, last pass to memory.

@@ -14,8 +14,8 @@ constexpr int CFL_SLOTS = 64;  // addresses the CFL maxima are spread over
 
 struct PlaneArgs {
   int Mx, Mz, Nx, Nz, Kx, C, nyu, nyv;
-  int w0, kRK;                     // first plane of this launch; RK substep
-  int wbase;                       // first plane held in I/P (the plane batch start)
+  int j0, nbj;                     // planes j0..j0+nbj-1 (of both kinds) in this batch
+  int kRK;
   double alp, bet;
   const double2 *u1, *u2, *u3;
   const double *yu, *yv;
@@ -53,7 +53,7 @@ struct ColArgs {
 bool plane_size_supported(int n);
 int xpass_rows_per_block(int Mx);  // rows of one x-pass block (largest variant); must divide Mz
 int plane_variants(int n);          // tile variants instantiated for transform length n
-// which: 0 z-inverse (any planes), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; planes [a.w0, a.w0+nw)
+// which: 0 z-inverse (nw = nbj + 1 field rows), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes (nw = nbj)
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);
 void launch_cfl_reduce(const double *slots, double *out, hipStream_t st);
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st);
@@ -70,7 +70,7 @@ void launch_solveP(const ColArgs &a, hipStream_t st);
 void launch_vcorr(const ColArgs &a, hipStream_t st);
 void launch_qx(const ColArgs &a, hipStream_t st);
 // fused path
-void launch_fused_up(const ColArgs &a, hipStream_t st);
+void launch_fused_up(const ColArgs &a, int variant, hipStream_t st);  // 0 thread/column, 1-2 wave/solve
 void launch_fused_down(const ColArgs &a, hipStream_t st);
 void launch_fused_final(const ColArgs &a, hipStream_t st);
 // layout

@@ -4,14 +4,19 @@
 // inverse real FFT in x on all Mz rows (rft), pointwise products, forward real FFT in x, forward FFT in z
 // on the Kx x-modes, the spectral derivatives der_x/der_z with their dealias mask, and keeps vv, uv, wv
 // for the y-flux.  Here that work is three kernels over a batch of planes:
-//   zinv_kernel : spectral plane (device layout [row][column]) -> I[plane][field][z][i]   (z-inverse)
-//                 The y-interpolations interp_u/interp_v (FOU3D.f90:373-418) are fused into its loads.
+//   zinv_kernel : spectral rows of one field -> I[plane][field][z][i]                    (z-inverse)
+//                 A workgroup walks the y-rows of a chunk of planes, transforms every field row once
+//                 and writes both the plain row (u1, u3 of u-plane j; u2 of v-plane j) and, from the
+//                 row and the previous one kept in registers, the y-interpolated row (interp_u/interp_v,
+//                 FOU3D.f90:373-418): the z-transform is linear, so interpolating after it equals
+//                 interpolating before it.
 //   xpass_kernel: I rows -> physical rows -> products -> P[plane][product][z][i]          (x-pass)
 //                 Also the CFL maxima of dtc_calc (FOU3D.f90:613-651) at kRK = 1.
 //   zfwd_kernel : P -> forward z FFT -> der_x + der_z (masked) and raw fluxes in device layout.
-// Plane index w: w < nyu is u-plane j = w+1 (u1, u2_itp, u3 -> uu, uw, vv, ww); otherwise v-plane
-// j = w-nyu+1 (u1_itp, u2, u3_itp -> uv, wv).  v-planes run to j = nyv+1 as the reference does on one
-// rank (start.f90:1128-1180 gives the single rank v-planes 1..nyu).
+// Planes: u-plane j (u1, u2_itp, u3 -> uu, uw, vv, ww) and v-plane j (u1_itp, u2, u3_itp -> uv, wv) for
+// j = 1..nyu.  v-planes run to j = nyv+1 as the reference does on one rank (start.f90:1128-1180 gives
+// the single rank v-planes 1..nyu).  A batch holds planes j0..j0+nbj-1 of both kinds: I and P plane
+// index jj (u) and nbj + jj (v), jj = j - j0.
 #pragma once
 #include "dns_kernels.h"
 
@@ -21,38 +26,48 @@ namespace dnsk {
 template <class PL, int N> constexpr int PL_M = N / plan_info<PL>::first;
 
 // ------------------------------------------------------------------------------------------------
-template <typename R, int MZ, int T, int NTH, class PL, int WPE = 1, int PS = 4>
+// Grid: (column tiles, field, chunk of ZCH planes); ZCH + 1 field rows per workgroup.  Field rows walked: u1, u3 rows jc0..jc1+1 (plain
+// -> u-plane r, interpolated with the row below -> v-plane r-1); u2 rows jc0-1..jc1 (plain -> v-plane r,
+// interpolated with the row below -> u-plane r).
+template <typename R, int MZ, int T, int NTH, class PL, int ZCH, int WPE = 1, int PS = 4>
 __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
   using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
   __shared__ cx<R> s[L::SIZE];
-  const int w = a.w0 + blockIdx.y;
+  constexpr int KS = last_slots<L, NTH, PL>();
+  const int f = blockIdx.y;
+  const int jc0 = a.j0 + blockIdx.z * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
   const int i0 = blockIdx.x * T;
-  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
-  const bool vpl = w >= a.nyu;
-  const int j = vpl ? w - a.nyu + 1 : w + 1;
+  const int Kx = a.Kx, C = a.C;
+  const size_t fsz = (size_t)MZ * Kx;
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
-  cx<R> *I = reinterpret_cast<cx<R> *>(a.I) + (size_t)(w - a.wbase) * 3 * MZ * Kx;
-  // interpolation weights (interp_v for u-planes, interp_u for v-planes): wa/dd on the upper plane,
-  // wb/dd on the lower one
-  double ca, cb;
-  if (!vpl) {
-    const double dd = a.yv[j] - a.yv[j - 1];
-    ca = (a.yu[j] - a.yv[j - 1]) / dd;
-    cb = (a.yv[j] - a.yu[j]) / dd;
-  } else {
-    const double dd = a.yu[j + 1] - a.yu[j];
-    ca = (a.yv[j] - a.yu[j]) / dd;
-    cb = (a.yu[j + 1] - a.yv[j]) / dd;
-  }
-  const int f = blockIdx.z;  // one field per block
-  {
-    // field f: on its own plane (hi only) or interpolated between planes lo and hi
-    const bool itp = vpl ? (f != 1) : (f == 1);
-    const double2 *fld = f == 0 ? a.u1 : f == 1 ? a.u2 : a.u3;
-    const double2 *hi = fld + (size_t)(vpl && itp ? j + 1 : j) * C;
-    const double2 *lo = fld + (size_t)(vpl ? j : j - 1) * C;
-    // position i + rM of column i0 + t; the rows of the 3/2 padding band (MZ/3 <= z < 2 MZ/3) are zero.
-    // When the first pass's stride divides MZ/3 the band is known per rM at compile time (no load).
+  const double2 *fld = f == 0 ? a.u1 : f == 1 ? a.u2 : a.u3;
+  cx<R> *I = reinterpret_cast<cx<R> *>(a.I) + (size_t)f * fsz + i0;
+  const int rbeg = f == 1 ? jc0 - 1 : jc0, rend = f == 1 ? jc1 : jc1 + 1;
+  cx<R> prev[KS];
+  for (int r = rbeg; r <= rend; r++) {
+    const double2 *row = fld + (size_t)r * C;
+    // plain output: u-plane r (u1, u3) or v-plane r (u2); interpolated output with row r-1:
+    // v-plane r-1 (u1, u3) or u-plane r (u2)
+    const bool plain = r >= jc0 && r <= jc1;
+    const int jp = f == 1 ? r : r - 1;
+    const bool itp = r > rbeg;
+    cx<R> *Ip = I + (size_t)((f == 1 ? a.nbj : 0) + (plain ? r - a.j0 : 0)) * 3 * fsz;
+    cx<R> *Ii = I + (size_t)((f == 1 ? 0 : a.nbj) + (itp ? jp - a.j0 : 0)) * 3 * fsz;
+    double ca_ = 0.0, cb_ = 0.0;  // weight of row r (upper) and row r-1 (lower)
+    if (itp) {
+      if (f == 1) {  // interp_v at yu[jp] between yv[jp-1] and yv[jp]
+        const double dd = a.yv[jp] - a.yv[jp - 1];
+        ca_ = (a.yu[jp] - a.yv[jp - 1]) / dd;
+        cb_ = (a.yv[jp] - a.yu[jp]) / dd;
+      } else {       // interp_u at yv[jp] between yu[jp] and yu[jp+1]
+        const double dd = a.yu[jp + 1] - a.yu[jp];
+        ca_ = (a.yv[jp] - a.yu[jp]) / dd;
+        cb_ = (a.yu[jp + 1] - a.yv[jp]) / dd;
+      }
+    }
+    const R ca = (R)ca_, cb = (R)cb_;
+    // position i + rM of column i0 + t; the rows of the 3/2 padding band (MZ/3 <= z < 2 MZ/3) are
+    // zero.  When the first pass's stride divides MZ/3 the band is known per rM at compile time.
     auto ld = [&](int i, auto rM, int t) -> cx<R> {
       constexpr int zr = decltype(rM)::value, B0 = MZ / 3, B1 = 2 * MZ / 3;
       constexpr int M = PL_M<PL, MZ>;
@@ -62,21 +77,19 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
       if (exact && !kept) return mk<R>(0, 0);
       const int ic = min(i0 + t, Kx - 1);             // clamped: loads stay in bounds, masked below
       const int zc = (exact ? zr < B0 : z < B0) ? z : z - B0;
-      const int col = zc * Kx + ic;
-      const double2 x1 = hi[col];
-      double2 v = x1;
-      if (itp) {
-        const double2 x0 = lo[col];
-        v = d2(ca * x1.x + cb * x0.x, ca * x1.y + cb * x0.y);
-      }
+      double2 v = row[zc * Kx + ic];
       if (!kept || i0 + t >= Kx) v = d2(0.0, 0.0);
       return from_d2<R>(v);
     };
-    cx<R> *If = I + (size_t)f * MZ * Kx + i0;
-    auto st = [&](int z, int t, const cx<R> &v) {
-      if (i0 + t < Kx) If[z * Kx + t] = v;
+    auto st = [&](int z, int t, int k, const cx<R> &v) {
+      if (i0 + t < Kx) {
+        if (plain) Ip[z * Kx + t] = v;
+        if (itp) Ii[z * Kx + t] = mk<R>(ca * v.x + cb * prev[k].x, ca * v.y + cb * prev[k].y);
+      }
+      prev[k] = v;
     };
     fft_mem<R, L, NTH, +1, PL>(s, tw, ld, st);
+    __syncthreads();
   }
 }
 
@@ -95,28 +108,31 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
   constexpr int H = MX / 2;
   constexpr int NS = RB * 4;
   constexpr int NP = VPL ? 2 : 4;
-  using LI = Lay<H, NS, RB * 3, 3, 4, PS, true>;   // inverse: slots r*4 + f, f < 3
+  using LI = Lay<H, NS, RB * 3, RB, 4, PS, true, true>;  // inverse: active f*RB + r -> slot r*4 + f
   using LF = Lay<H, NS, RB * NP, NP, 4, PS, true>;  // forward: slots r*4 + p, p < NP
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
-  const int w = a.w0 + blockIdx.y;
+  const int jj = blockIdx.y, j = a.j0 + jj;
   const int z0 = blockIdx.x * RB;
-  const int Kx = a.Kx, MZ = a.Mz;
-  const int j = VPL ? w - a.nyu + 1 : w + 1;
+  constexpr int Kx = MX / 3 + 1;   // retained x-modes Nx/2 + 1, Nx = 2 MX/3 (the launcher checks a.Kx)
+  const int MZ = a.Mz;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
   const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}
-  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + (size_t)(w - a.wbase) * 3 * MZ * Kx;
-  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + (size_t)(w - a.wbase) * 4 * MZ * Kx;
+  const size_t fsz = (size_t)MZ * Kx;
+  const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);
+  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + pl * 3 * fsz;
+  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + pl * 4 * fsz;
 
-  // packed half-length spectrum Z_n of field f of row z0 + r (active transform q = r*3 + f)
+  // packed half-length spectrum Z_n of field f of row z0 + r (active transform q = f*RB + r)
   auto ld = [&](int i, auto rM, int q) -> cx<R> {
     const int n = i + decltype(rM)::value;
-    const int r = q / 3, f = q - r * 3;
-    const cx<R> *row = I + ((size_t)f * MZ + (z0 + r)) * Kx;
+    const int f = q / RB, r = q - f * RB;
+    const cx<R> *row = I + f * fsz + (size_t)(z0 + r) * Kx;
+    auto X = [&](int m) -> cx<R> { return row[m]; };
     cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
-    if (n < Kx) xa = row[n];
+    if (n < Kx) xa = X(n);
     if (n == 0) xa.y = 0;
-    if (n > 0 && H - n < Kx) xb = row[H - n];
+    if (n > 0 && H - n < Kx) xb = X(H - n);
     const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
     const cx<R> e = cconj(twx[n]);                        // e^{+2 pi i n/MX}
     return cadd(sum, cmul_si<1>(cmul(dif, e)));
@@ -184,12 +200,12 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
   __shared__ cx<R> s[L::SIZE];
   constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
   constexpr int NP = VPL ? 2 : 4;
-  const int w = a.w0 + blockIdx.y;
+  const int jj = blockIdx.y, j = a.j0 + jj;
   const int i0 = blockIdx.x * T;
-  const int Kx = a.Kx, Nz = a.Nz, C = a.C;
-  const int j = VPL ? w - a.nyu + 1 : w + 1;
+  constexpr int Nz = 2 * MZ / 3;    // retained z-modes (the launcher checks a.Nz)
+  const int Kx = a.Kx, C = a.C;
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
-  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)(w - a.wbase) * 4 * MZ * Kx;
+  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)((VPL ? a.nbj : 0) + jj) * 4 * MZ * Kx;
   const double inv = 1.0 / (double)MZ;
   const size_t ju = (size_t)(j - 1) * C, jf = (size_t)j * C;
   double2 accA[PERMAX], accB[PERMAX];

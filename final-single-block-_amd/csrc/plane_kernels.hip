@@ -141,7 +141,8 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
 // launchers.  The radix plan is fixed per transform length (so every tile variant of a kernel gives the
 // same bits); the tile variants differ in columns/rows per workgroup and threads, and the autotuner
 // picks among them.  Plans and tiles of the tuned lengths were chosen by timing on MI355X:
-//   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column);
+//   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column),
+//     16 columns per workgroup (256-byte row segments in FP64);
 //   x-pass half-length 288: 16.18 (two passes).
 #define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
 #define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
@@ -155,12 +156,13 @@ template <> struct XPlanT<576> { using type = Plan<24, 24>; };
 
 constexpr size_t LDS_MAX = 160 * 1024;
 
-// z-kernel tiles (T columns, NTH threads); 12.12.x plans: 48 threads per column (one butterfly each),
-// or 24 (two butterflies each).
+// z-kernel tiles (T columns, NTH threads); 12.12.x plans: 48 threads per column (one butterfly each).
 template <int N, int V> struct ZTile {
   static constexpr bool p12 = (N == 576 || N == 1152);
-  static constexpr int T = p12 ? (V == 1 ? 4 : 8) : (V == 0 ? 8 : 4);
-  static constexpr int NTH = p12 ? (V == 2 ? 24 * T : 48 * T) : (V == 2 ? 128 : 256);
+  static constexpr int T = N == 576 ? (V == 0 ? 16 : V == 1 ? 8 : 4)
+                         : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 8 : 4);
+  static constexpr int NTH = p12 ? 48 * T : (V == 2 ? 128 : 256);
+  static constexpr int ZCH = 2;  // planes per z-inverse workgroup (ZCH + 1 field rows each)
 };
 // x-pass tiles (RB rows, NTH threads)
 template <int MX, int V> struct XTile {
@@ -185,8 +187,9 @@ static void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
   if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
     zinv_v<R, MZ, 0>(a, nw, st);
   } else {
-    dim3 g((a.Kx + Z::T - 1) / Z::T, nw, 3);
-    zinv_kernel<R, MZ, Z::T, Z::NTH, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
+    (void)nw;
+    dim3 g((a.Kx + Z::T - 1) / Z::T, 3, (a.nbj + Z::ZCH - 1) / Z::ZCH);
+    zinv_kernel<R, MZ, Z::T, Z::NTH, typename ZPlanT<MZ>::type, Z::ZCH, 1, 0><<<g, Z::NTH, 0, st>>>(a);
   }
 }
 template <typename R, int MZ, int V, bool VPL>
@@ -239,7 +242,7 @@ static void xpass_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
 }
 
 // which: 0 z-inverse, 1 x-pass u-planes, 2 x-pass v-planes, 3 z-forward u-planes, 4 z-forward v-planes.
-// The caller passes planes [a.w0, a.w0 + nw) of the right type for which = 1..4.
+// The batch is a.j0, a.nbj; nw = blockIdx.y extent (a.nbj + 1 field rows for the z-inverse, else a.nbj).
 template <typename R>
 static int launch_t(const PlaneArgs &a, int nw, int which, int v, hipStream_t st) {
   const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
@@ -259,6 +262,7 @@ static int launch_t(const PlaneArgs &a, int nw, int which, int v, hipStream_t st
 }
 
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st) {
+  if (a.Kx != a.Mx / 3 + 1 || a.Nz != 2 * a.Mz / 3) return -1;  // the kernels take both from Mx, Mz
   return fp32 ? launch_t<float>(a, nw, which, variant, st) : launch_t<double>(a, nw, which, variant, st);
 }
 

@@ -215,27 +215,28 @@ class DNS:
                     substep=t[6], n=n.value)
 
     # ---- plane-kernel tile variants and the autotuner
-    KERNELS = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v")
+    KERNELS = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v", "col_up", "col_down", "col_final")
+    NKERN = 8
 
     def plane_variants(self):
-        """(current variant per kernel, number of variants per kernel), both int arrays of 5."""
-        v = np.zeros(5, np.int32)
-        c = np.zeros(5, np.int32)
+        """(current variant per kernel, number of variants per kernel), int arrays of NKERN."""
+        v = np.zeros(self.NKERN, np.int32)
+        c = np.zeros(self.NKERN, np.int32)
         _ip = ctypes.POINTER(ctypes.c_int)
         _chk(lib().dns_get_plane_variants(self.h, v.ctypes.data_as(_ip), c.ctypes.data_as(_ip)))
         return v, c
 
     def set_plane_variants(self, var):
         v = np.ascontiguousarray(var, np.int32)
-        assert v.shape == (5,)
+        assert v.shape == (self.NKERN,)
         _chk(lib().dns_set_plane_variants(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
 
     def autotune(self, iters=5):
-        """Times every variant; returns (chosen[5], ms[5, 3]) with -1 for missing variants."""
-        ch = np.zeros(5, np.int32)
-        ms = np.zeros(15)
+        """Times every variant; returns (chosen[NKERN], ms[NKERN, 3]) with -1 where not timed."""
+        ch = np.zeros(self.NKERN, np.int32)
+        ms = np.zeros(3 * self.NKERN)
         _chk(lib().dns_autotune(self.h, iters, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms)))
-        return ch, ms.reshape(5, 3)
+        return ch, ms.reshape(self.NKERN, 3)
 
     # ---- device-pointer utilities (torch tensors provide the device memory)
     def plane_fft_roundtrip(self, d_in, d_phys, d_back, n):

@@ -179,19 +179,19 @@ module dns_hip
     integer(c_int) function dns_get_plane_variants(h, var, count) bind(C, name='dns_get_plane_variants')
       import :: c_int, c_ptr
       type(c_ptr), value :: h
-      integer(c_int) :: var(5), count(5)
+      integer(c_int) :: var(8), count(8)
     end function
     integer(c_int) function dns_set_plane_variants(h, var) bind(C, name='dns_set_plane_variants')
       import :: c_int, c_ptr
       type(c_ptr), value :: h
-      integer(c_int), intent(in) :: var(5)
+      integer(c_int), intent(in) :: var(8)
     end function
     integer(c_int) function dns_autotune(h, iters, chosen, ms) bind(C, name='dns_autotune')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       integer(c_int), value :: iters
-      integer(c_int) :: chosen(5)
-      real(c_double) :: ms(15)
+      integer(c_int) :: chosen(8)
+      real(c_double) :: ms(24)
     end function
     function c_strlen(s) bind(C, name='strlen')
       import :: c_ptr, c_size_t

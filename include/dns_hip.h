@@ -46,7 +46,8 @@ typedef struct dns_params {
   double mpgx;            /* mpgx (input.in:38) */
   double dyq;             /* dyq (input.in:20) */
   double fixed_dt;        /* > 0: use this dt instead of the reference's CFL rule (FOU3D.f90:290-303) */
-  int plane_batch;        /* planes per plane-stage launch group; 0 = all planes (autotuner sets it) */
+  int plane_batch;        /* planes per plane-stage launch group (u- and v-planes counted, rounded up to
+                             an even number); 0 = all planes */
   int reserved[7];
 } dns_params;
 
@@ -130,19 +131,22 @@ int dns_sync(dns_handle *h);
 int dns_set_timing(dns_handle *h, int on);
 int dns_get_timing(dns_handle *h, double *t, int *n);
 
-/* Tile variants of the five plane kernels, index c: 0 z-inverse, 1 x-pass of u-planes, 2 x-pass of
- * v-planes, 3 z-forward of u-planes, 4 z-forward of v-planes.  All variants of a kernel give
- * bit-identical results.  No counterpart in the reference (FFTPACK has one code path); this replaces
- * the reference's fixed choice of loop order in four_to_phys_du / phys_to_four_du (FOU3D.f90:857-1231).
- * dns_get_plane_variants: var[5] current choice, count[5] (may be NULL) variants available.
- * dns_set_plane_variants: var[5]; status 1 for a variant that does not exist. */
+/* Kernel variants, index c: 0 z-inverse, 1 x-pass of u-planes, 2 x-pass of v-planes, 3 z-forward of
+ * u-planes, 4 z-forward of v-planes (tile shapes), 5 column sweep up (thread per column or one
+ * wavefront per solve), 6 column sweep down, 7 final sweep.  All variants of a kernel give
+ * bit-identical results.  No counterpart in the reference (one code path per subroutine); they stand
+ * in for the loop orders of four_to_phys_du / phys_to_four_du (FOU3D.f90:857-1231) and of the solves
+ * (tridLU_3D.f90:38-221).
+ * dns_get_plane_variants: var[DNS_NKERN] current choice, count[DNS_NKERN] (may be NULL) variants available.
+ * dns_set_plane_variants: var[DNS_NKERN]; status 1 for a variant that does not exist. */
+#define DNS_NKERN 8
 int dns_get_plane_variants(dns_handle *h, int *var, int *count);
 int dns_set_plane_variants(dns_handle *h, const int *var);
-/* Autotuner: times each variant of each plane kernel (iters launches over the first plane batch) and
- * keeps the fastest.  chosen[5] (may be NULL) receives the choice, ms[15] (may be NULL) the time per
- * launch of variant v of kernel c at ms[3c+v] (-1 where the variant does not exist).  Rewrites the
- * plane-stage scratch (nonlinear terms and fluxes of the current substep): call it before a step or
- * between steps.  Drops a captured graph (the next dns_rk3_step_graph captures the new choice). */
+/* Autotuner: times each variant of each kernel that has several (iters launches; plane kernels on the
+ * first plane batch) and keeps the fastest.  chosen[DNS_NKERN] (may be NULL) receives the choice,
+ * ms[3*DNS_NKERN] (may be NULL) the time per launch of variant v of kernel c at ms[3c+v] (-1 where not
+ * timed).  Rewrites scratch only (the fields are saved and restored): call it before a step or between
+ * steps.  Drops a captured graph (the next dns_rk3_step_graph captures the new choice). */
 int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms);
 
 #ifdef __cplusplus
