@@ -245,7 +245,8 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   rc |= chk(dalloc(h, &h->stage, C * ru));
   rc |= chk(dalloc(h, &h->st, 1));
   const size_t rsz = h->fp32 ? sizeof(float2) : sizeof(double2);
-  const size_t iel = (size_t)2 * h->bj * 3 * Mz * h->Kx, pel = (size_t)2 * h->bj * 4 * Mz * h->Kx;
+  const int Kp = h->fp32 ? dnsk::row_pitch<float>(h->Kx) : dnsk::row_pitch<double>(h->Kx);
+  const size_t iel = (size_t)2 * h->bj * 3 * Mz * Kp, pel = (size_t)2 * h->bj * 4 * Mz * Kp;
   char *Ib = nullptr, *Pb = nullptr, *tz = nullptr, *tx = nullptr, *th = nullptr;
   rc |= chk(dalloc(h, &Ib, iel * rsz)); rc |= chk(dalloc(h, &Pb, pel * rsz));
   rc |= chk(dalloc(h, &tz, (size_t)Mz * rsz)); rc |= chk(dalloc(h, &tx, (size_t)Mx * rsz));

@@ -12,6 +12,13 @@ namespace dnsk {
 //   Nu2, Nxz2 : rows 1..nyv at (j-1)*C
 constexpr int CFL_SLOTS = 64;  // addresses the CFL maxima are spread over
 
+// Row pitch (complex elements of precision R) of the plane intermediates I and P: Kx rounded up to
+// whole 128-byte lines, so that the tile segments of a row start on a line.
+template <typename R> __host__ __device__ constexpr int row_pitch(int kx) {
+  constexpr int e = (int)(128 / (2 * sizeof(R)));
+  return (kx + e - 1) / e * e;
+}
+
 struct PlaneArgs {
   int Mx, Mz, Nx, Nz, Kx, C, nyu, nyv;
   int j0, nbj;                     // planes j0..j0+nbj-1 (of both kinds) in this batch

@@ -37,8 +37,8 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
   const int f = blockIdx.y;
   const int jc0 = a.j0 + blockIdx.z * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
   const int i0 = blockIdx.x * T;
-  const int Kx = a.Kx, C = a.C;
-  const size_t fsz = (size_t)MZ * Kx;
+  const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);
+  const size_t fsz = (size_t)MZ * Kp;
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
   const double2 *fld = f == 0 ? a.u1 : f == 1 ? a.u2 : a.u3;
   cx<R> *I = reinterpret_cast<cx<R> *>(a.I) + (size_t)f * fsz + i0;
@@ -83,8 +83,8 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
     };
     auto st = [&](int z, int t, int k, const cx<R> &v) {
       if (i0 + t < Kx) {
-        if (plain) Ip[z * Kx + t] = v;
-        if (itp) Ii[z * Kx + t] = mk<R>(ca * v.x + cb * prev[k].x, ca * v.y + cb * prev[k].y);
+        if (plain) Ip[z * Kp + t] = v;
+        if (itp) Ii[z * Kp + t] = mk<R>(ca * v.x + cb * prev[k].x, ca * v.y + cb * prev[k].y);
       }
       prev[k] = v;
     };
@@ -118,7 +118,8 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
   const int MZ = a.Mz;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
   const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}
-  const size_t fsz = (size_t)MZ * Kx;
+  constexpr int Kp = row_pitch<R>(Kx);
+  const size_t fsz = (size_t)MZ * Kp;
   const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);
   const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + pl * 3 * fsz;
   cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + pl * 4 * fsz;
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
   auto ld = [&](int i, auto rM, int q) -> cx<R> {
     const int n = i + decltype(rM)::value;
     const int f = q / RB, r = q - f * RB;
-    const cx<R> *row = I + f * fsz + (size_t)(z0 + r) * Kx;
+    const cx<R> *row = I + f * fsz + (size_t)(z0 + r) * Kp;
     auto X = [&](int m) -> cx<R> { return row[m]; };
     cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
     if (n < Kx) xa = X(n);
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
     cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
     x = cscale(x, sc);
     if (k == 0) x.y = 0;
-    P[((size_t)p * MZ + (z0 + r)) * Kx + k] = x;
+    P[((size_t)p * MZ + (z0 + r)) * Kp + k] = x;
   }
 }
 
@@ -203,9 +204,9 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
   const int jj = blockIdx.y, j = a.j0 + jj;
   const int i0 = blockIdx.x * T;
   constexpr int Nz = 2 * MZ / 3;    // retained z-modes (the launcher checks a.Nz)
-  const int Kx = a.Kx, C = a.C;
+  const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
-  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)((VPL ? a.nbj : 0) + jj) * 4 * MZ * Kx;
+  const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)((VPL ? a.nbj : 0) + jj) * 4 * MZ * Kp;
   const double inv = 1.0 / (double)MZ;
   const size_t ju = (size_t)(j - 1) * C, jf = (size_t)j * C;
   double2 accA[PERMAX], accB[PERMAX];
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
     auto ld = [&](int i, auto rM, int t) -> cx<R> {
       const int z = i + decltype(rM)::value;
       const int ic = min(i0 + t, Kx - 1);             // clamped: loads stay in bounds, masked below
-      const cx<R> v = P[((size_t)p * MZ + z) * Kx + ic];
+      const cx<R> v = P[((size_t)p * MZ + z) * Kp + ic];
       return i0 + t < Kx ? v : mk<R>(0, 0);
     };
     fft_from_mem<R, L, NTH, -1, PL>(s, tw, ld);
