@@ -598,6 +598,175 @@ __global__ void __launch_bounds__(256) fused_down_kernel(ColArgs a) {
   }
 }
 
+// The down sweep on four wavefronts of one workgroup (64 columns): waves 0, 1, 2 back-substitute u1, u3,
+// u2 and hand each chunk of K rows to wave 3 through double-buffered LDS; wave 3 forms the divergence and
+// eliminates the pressure system one chunk behind.  Arithmetic identical to fused_down_kernel.
+template <int K>
+__global__ void __launch_bounds__(256) fused_down4_kernel(ColArgs a) {
+  __shared__ double2 buf[2][3][K][64];
+  const int f = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 64 + lane;
+  const bool act = col < a.C;
+  const int C = a.C, nyu = a.nyu;
+  const int nch = (nyu + K - 1) / K;  // chunk c: rows j = nyu - c K - k, k < K, j >= 1
+  double2 U = d2(0.0, 0.0);
+  double2 *F = f == 0 ? a.u1 : f == 1 ? a.u3 : a.u2;
+  if (act && f < 3) U = F[(size_t)(f < 2 ? nyu + 1 : nyu) * C + col];  // u rows nyu+1; v row nyv+1
+  // pressure wave state
+  Mode m = col_mode(a, act ? col : 0);
+  const double dtirk = a.st->dti / (a.aRK + a.bRK);
+  double2 xp = d2(0.0, 0.0), Vj = act ? a.u2[(size_t)nyu * C + col] : d2(0.0, 0.0);
+  double cprev = 0.0;
+  for (int c = 0; c <= nch; c++) {
+    if (f < 3 && c < nch && act) {
+      for (int k = 0; k < K; k++) {
+        const int j = nyu - c * K - k;
+        if (j < 1) break;
+        const int row = f < 2 ? j : j - 1;  // u row j, v row j-1
+        const double2 y = F[(size_t)row * C + col];
+        const double cc = (f < 2 ? a.w3u : a.w3v)[(size_t)row * C + col];
+        U = d2(y.x - cc * U.x, y.y - cc * U.y);
+        F[(size_t)row * C + col] = U;
+        buf[c & 1][f][k][lane] = U;
+      }
+    }
+    if (f == 3 && c > 0 && act) {
+      const int cc = c - 1;
+      for (int k = 0; k < K; k++) {
+        const int j = nyu - cc * K - k;
+        if (j < 1) break;
+        const double2 U1 = buf[cc & 1][0][k][lane], U3 = buf[cc & 1][1][k][lane], V = buf[cc & 1][2][k][lane];
+        const double2 dv = cadd(cadd(imul(m.kx, U1), imul(m.kz, U3)), rmul(1.0 * a.dthdyu[j], csub(Vj, V)));
+        double2 r = rmul(dtirk, dv);
+        if (col == 0 && j == 1) r = d2(0.0, 0.0);
+        double a1, a2, a3;
+        dg_row(a, col, m.k2x, m.k2z, j, a1, a2, a3);
+        const double piv = (j == nyu) ? a2 : a2 - a3 * cprev;
+        const double ip = 1.0 / piv;
+        xp = (j == nyu) ? rmul(ip, r) : d2((r.x - a3 * xp.x) * ip, (r.y - a3 * xp.y) * ip);
+        cprev = a1 * ip;
+        a.psi[(size_t)(j - 1) * C + col] = xp;
+        a.cP[(size_t)(j - 1) * C + col] = cprev;
+        Vj = V;
+      }
+    }
+    __syncthreads();
+  }
+  if (f < 2 && act) {  // u row 0
+    const double2 y = F[col];
+    const double cc = a.w3u[col];
+    U = d2(y.x - cc * U.x, y.y - cc * U.y);
+    F[col] = U;
+  }
+}
+
+// The final sweep on four wavefronts (64 columns): wave 0 back-substitutes the pressure correction
+// and updates p; waves 1, 2, 3 apply the projection to u1, u3, u2 one chunk of K rows behind; wave 0
+// forms the divergence of the projected velocities one further chunk behind.  Arithmetic identical to
+// fused_final_kernel.
+template <int K>
+__global__ void __launch_bounds__(256) fused_final4_kernel(ColArgs a) {
+  __shared__ double2 psb[2][K][64];     // ps of rows j (chunk parity)
+  __shared__ double2 vb[2][3][K][64];   // projected u1(j), u3(j), v(j-1)
+  __shared__ double red[4];
+  const int f = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 64 + lane;
+  const bool act = col < a.C;
+  const int C = a.C, nyu = a.nyu, nyv = a.nyv;
+  const int nch = (nyu + K - 1) / K;  // chunk c: rows j = 1 + c K + k <= nyu
+  const Mode m = col_mode(a, act ? col : 0);
+  const double dtrk = a.st->dt * (a.aRK + a.bRK);
+  const double kxd = dtrk * m.kx, kzd = dtrk * m.kz;
+  double dmax = 0.0;
+  double2 psm = d2(0.0, 0.0);                      // wave 0: previous ps; wave 3: ps(j-1)
+  double2 last = d2(0.0, 0.0);                     // waves 1, 2: last projected u row
+  double Q = 0.0;
+  const bool qx = (col == 0) && (a.kRK == 3) && (a.flag_ctpress == 1);
+  double2 u1prev = d2(0.0, 0.0), u3prev = d2(0.0, 0.0), v_m2 = d2(0.0, 0.0);  // wave 0, divergence
+  for (int c = 0; c <= nch + 1; c++) {
+    if (f == 0 && act) {
+      if (c < nch) {  // pressure back substitution of chunk c
+        for (int k = 0; k < K; k++) {
+          const int j = 1 + c * K + k;
+          if (j > nyu) break;
+          const size_t o = (size_t)(j - 1) * C + col;
+          const double2 xj = a.psi[o];
+          const double2 ps = (j == 1) ? xj : csub(xj, rmul(a.cP[o], psm));
+          a.p[o] = cadd(a.p[o], ps);
+          psb[c & 1][k][lane] = ps;
+          psm = ps;
+        }
+      }
+      if (c >= 2) {  // divergence of chunk c-2 (centres j-1)
+        const int cc = c - 2;
+        for (int k = 0; k < K; k++) {
+          const int j = 1 + cc * K + k;
+          if (j > nyu) break;
+          const double2 u1n = vb[cc & 1][0][k][lane], u3n = vb[cc & 1][1][k][lane];
+          if (j >= 2) {
+            const double2 v_m1 = vb[cc & 1][2][k][lane];
+            const double2 dv = cadd(cadd(imul(m.kx, u1prev), imul(m.kz, u3prev)),
+                                    rmul(1.0 * a.dthdyu[j - 1], csub(v_m1, v_m2)));
+            dmax = fmax(dmax, hypot(dv.x, dv.y));
+            v_m2 = v_m1;
+          }
+          u1prev = u1n;
+          u3prev = u3n;
+        }
+      }
+    }
+    if (f > 0 && act && c >= 1 && c <= nch) {  // projection of chunk c-1
+      const int cc = c - 1;
+      for (int k = 0; k < K; k++) {
+        const int j = 1 + cc * K + k;
+        if (j > nyu) break;
+        const double2 ps = psb[cc & 1][k][lane];
+        if (f == 1 || f == 2) {
+          double2 *U = f == 1 ? a.u1 : a.u3;
+          const double kd = f == 1 ? kxd : kzd;
+          const double2 un = csub(U[(size_t)j * C + col], imul(kd, ps));
+          U[(size_t)j * C + col] = un;
+          if (j == 1) U[col] = rmul(a.gbc, un);
+          if (f == 1 && qx) Q = Q + (a.yv[j] - a.yv[j - 1]) * un.x;
+          vb[cc & 1][f - 1][k][lane] = un;
+          last = un;
+        } else {
+          double2 vn = d2(0.0, 0.0);
+          if (j == 1) {
+            a.u2[col] = d2(0.0, 0.0);
+          } else {  // v row j-1 in 1..nyv
+            vn = csub(a.u2[(size_t)(j - 1) * C + col], rmul(dtrk * 1.0 * a.dthdyv[j - 1], csub(ps, psm)));
+            a.u2[(size_t)(j - 1) * C + col] = vn;
+          }
+          vb[cc & 1][2][k][lane] = vn;
+          psm = ps;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (act) {
+    if (f == 1) {
+      a.u1[(size_t)(nyu + 1) * C + col] = rmul(a.gbc, last);
+      if (qx) a.st->Qx = Q;
+      if (col == 0 && a.kRK == 3) a.st->iter = a.st->iter + 1.0;
+    } else if (f == 2) {
+      a.u3[(size_t)(nyu + 1) * C + col] = rmul(a.gbc, last);
+    } else if (f == 3) {
+      a.u2[(size_t)(nyv + 1) * C + col] = d2(0.0, 0.0);
+    } else {
+      const double2 dv = cadd(cadd(imul(m.kx, u1prev), imul(m.kz, u3prev)),
+                              rmul(1.0 * a.dthdyu[nyu], csub(d2(0.0, 0.0), v_m2)));
+      dmax = fmax(dmax, hypot(dv.x, dv.y));
+    }
+  }
+  // block max over wave 0's lanes (the other waves hold 0)
+  for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+  if (lane == 0) red[f] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) atomic_max_nonneg(&a.st->divmax, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+}
+
 template <int NTH>
 __global__ void __launch_bounds__(NTH) fused_final_kernel(ColArgs a) {
   __shared__ double red[NTH / 64];
@@ -718,9 +887,15 @@ void launch_fused_up(const ColArgs &a, int variant, hipStream_t st) {
   else if (variant == 2) fused_up3_kernel<true><<<nblk(a.C, 64), 192, 0, st>>>(a);
   else fused_up_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a);
 }
-void launch_fused_down(const ColArgs &a, hipStream_t st) { fused_down_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_fused_final(const ColArgs &a, hipStream_t st) {
-  fused_final_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a);
+void launch_fused_down(const ColArgs &a, int variant, hipStream_t st) {
+  if (variant == 1) fused_down4_kernel<4><<<nblk(a.C, 64), 256, 0, st>>>(a);
+  else if (variant == 2) fused_down4_kernel<8><<<nblk(a.C, 64), 256, 0, st>>>(a);
+  else fused_down_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a);
+}
+void launch_fused_final(const ColArgs &a, int variant, hipStream_t st) {
+  if (variant == 1) fused_final4_kernel<4><<<nblk(a.C, 64), 256, 0, st>>>(a);
+  else if (variant == 2) fused_final4_kernel<8><<<nblk(a.C, 64), 256, 0, st>>>(a);
+  else fused_final_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a);
 }
 
 void launch_ref_to_dev(const double2 *ref, double2 *dev, int C, int nrows, hipStream_t st) {

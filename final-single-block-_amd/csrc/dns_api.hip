@@ -618,10 +618,10 @@ static int fused_substep(dns_handle *h, int kRK) {
   ColArgs a = col_args(h, kRK);
   dnsk::launch_fused_up(a, h->var[5], h->stream);
   tmark(h, 4);
-  dnsk::launch_fused_down(a, h->stream);
+  dnsk::launch_fused_down(a, h->var[6], h->stream);
   tmark(h, 5);
   if (h->prm.flag_ctpress == 0) dnsk::launch_meanflow(a, h->stream);
-  dnsk::launch_fused_final(a, h->stream);
+  dnsk::launch_fused_final(a, h->var[7], h->stream);
   tmark(h, 6);
   HIPCHK(hipGetLastError());
   return DNS_OK;
@@ -682,7 +682,7 @@ extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
 static int kern_len(const dns_handle *h, int c) { return (c == 1 || c == 2) ? h->Mx : h->Mz; }
 static int kern_variants(const dns_handle *h, int c) {
   if (c < 5) return dnsk::plane_variants(kern_len(h, c));
-  return c == 5 ? 3 : 1;
+  return 3;
 }
 
 static void drop_graph(dns_handle *h) {
@@ -724,17 +724,19 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  // the column sweep overwrites u1, u2, u3 and Nu: keep copies and put them back afterwards
+  // the column sweeps overwrite u1, u2, u3, p and Nu: keep copies and put them back afterwards
   const size_t C = h->C;
   const size_t nu = C * (h->nyu + 2), nv = C * (h->nyv + 2), nn = C * h->nyu;
   double2 *save = nullptr;
   int rc = DNS_OK;
-  if (kern_variants(h, 5) > 1) {
-    HIPCHK(hipMalloc(&save, (2 * nu + nv + 3 * nn) * sizeof(double2)));
+  const double2 *src[7] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3, h->p};
+  const size_t len[7] = {nu, nu, nv, nn, nn, nn, nn};
+  DnsState st_save;
+  {
+    HIPCHK(hipMalloc(&save, (2 * nu + nv + 4 * nn) * sizeof(double2)));
+    HIPCHK(hipMemcpyAsync(&st_save, h->st, sizeof st_save, hipMemcpyDeviceToHost, h->stream));
     double2 *q = save;
-    const double2 *src[6] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3};
-    const size_t len[6] = {nu, nu, nv, nn, nn, nn};
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < 7; k++) {
       HIPCHK(hipMemcpyAsync(q, src[k], len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
       q += len[k];
     }
@@ -750,7 +752,9 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
       if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0)) continue;
       auto launch = [&]() -> int {
         if (c < 5) return dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
-        dnsk::launch_fused_up(ca, v, h->stream);
+        if (c == 5) dnsk::launch_fused_up(ca, v, h->stream);
+        else if (c == 6) dnsk::launch_fused_down(ca, v, h->stream);
+        else dnsk::launch_fused_final(ca, v, h->stream);
         return 0;
       };
       if (launch()) { rc = fail(DNS_ERR_ARG, "dns_autotune: size not instantiated"); break; }
@@ -770,12 +774,12 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   }
   if (save) {
     double2 *q = save;
-    double2 *dst[6] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3};
-    const size_t len[6] = {nu, nu, nv, nn, nn, nn};
-    for (int k = 0; k < 6; k++) {
-      (void)hipMemcpyAsync(dst[k], q, len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream);
+    for (int k = 0; k < 7; k++) {
+      (void)hipMemcpyAsync(const_cast<double2 *>(src[k]), q, len[k] * sizeof(double2), hipMemcpyDeviceToDevice,
+                           h->stream);
       q += len[k];
     }
+    (void)hipMemcpyAsync(h->st, &st_save, sizeof st_save, hipMemcpyHostToDevice, h->stream);
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(save);
   }
