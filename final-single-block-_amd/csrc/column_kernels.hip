@@ -115,15 +115,15 @@ __global__ void nl_col_kernel(ColArgs a) {
     const double2 dy3 = rmul(a.dthdyu[j], csub(a.wvf[(size_t)j * C + col], a.wvf[(size_t)(j - 1) * C + col]));
     const double2 N1 = cadd(a.Nxz1[(size_t)(j - 1) * C + col], dy1);
     const double2 N3 = cadd(a.Nxz3[(size_t)(j - 1) * C + col], dy3);
-    a.Nu1[(size_t)(j - 1) * C + col] = N1;
-    a.Nu3[(size_t)(j - 1) * C + col] = N3;
+    a.Nu1.set((size_t)(j - 1) * C + col, N1);
+    a.Nu3.set((size_t)(j - 1) * C + col, N3);
     a.du1[(size_t)j * C + col] = cadd(a.du1[(size_t)j * C + col], rmul(Cg, N1));
     a.du3[(size_t)j * C + col] = cadd(a.du3[(size_t)j * C + col], rmul(Cg, N3));
   }
   for (int j = 1; j <= a.nyv; j++) {
     const double2 dy2 = rmul(a.dthdyv[j], csub(a.vvc[(size_t)(j + 1) * C + col], a.vvc[(size_t)j * C + col]));
     const double2 N2 = cadd(a.Nxz2[(size_t)(j - 1) * C + col], dy2);
-    a.Nu2[(size_t)(j - 1) * C + col] = N2;
+    a.Nu2.set((size_t)(j - 1) * C + col, N2);
     a.du2[(size_t)j * C + col] = cadd(a.du2[(size_t)j * C + col], rmul(Cg, N2));
   }
 }
@@ -379,8 +379,8 @@ __global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a) {
     const double2 N3 = cadd(a.Nxz3[(size_t)(j - 1) * C + col], rmul(a.dthdyu[j], csub(wvc, wvm)));
     r1 = cadd(r1, rmul(Cg, N1));
     r3 = cadd(r3, rmul(Cg, N3));
-    a.Nu1[(size_t)(j - 1) * C + col] = N1;
-    a.Nu3[(size_t)(j - 1) * C + col] = N3;
+    a.Nu1.set((size_t)(j - 1) * C + col, N1);
+    a.Nu3.set((size_t)(j - 1) * C + col, N3);
     const double2 rhs1 = d2(u1c.x + dt * r1.x, u1c.y + dt * r1.y);
     const double2 rhs3 = d2(u3c.x + dt * r3.x, u3c.y + dt * r3.y);
     {
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a) {
       double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, a.Nu2[(size_t)(j - 1) * C + col])), rmul(C3, csub(pn, pc)));
       const double2 N2 = cadd(a.Nxz2[(size_t)(j - 1) * C + col], rmul(a.dthdyv[j], csub(vvn, vvc_c)));
       r2 = cadd(r2, rmul(Cg, N2));
-      a.Nu2[(size_t)(j - 1) * C + col] = N2;
+      a.Nu2.set((size_t)(j - 1) * C + col, N2);
       const double2 rhs2 = d2(u2c.x + dt * r2.x, u2c.y + dt * r2.y);
       const double a1 = -beta * D2V(1, j), a2 = 1.0 - beta * (D2V(2, j) + m.k2x + m.k2z), a3 = -beta * D2V(3, j);
       w2 = 1.0 / (a2 - a1 * w3v);
@@ -451,9 +451,9 @@ __global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
   double w2 = 1.0 / 1.0;
   if (f < 2) {
     double2 *U = f == 0 ? a.u1 : a.u3;
-    double2 *Nu = f == 0 ? a.Nu1 : a.Nu3;
-    const double2 *Nxz = f == 0 ? a.Nxz1 : a.Nxz3;
-    const double2 *fl = f == 0 ? a.uvf : a.wvf;
+    const NArr Nu = f == 0 ? a.Nu1 : a.Nu3;
+    const NArr Nxz = f == 0 ? a.Nxz1 : a.Nxz3;
+    const NArr fl = f == 0 ? a.uvf : a.wvf;
     const double c3 = f == 0 ? -(a.dRK * m.kx) : -(a.dRK * m.kz);
     const double Cm = col == 0 ? -(a.dRK * a.st->mpgx) : 0.0;
     double2 um = U[col], uc = U[(size_t)C + col];
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
       if (f == 0) r.x = r.x + Cm;
       const double2 N = cadd(cur.nx, rmul(a.dthdyu[j], csub(cur.flc, flm)));
       r = cadd(r, rmul(Cg, N));
-      Nu[(size_t)(j - 1) * C + col] = N;
+      Nu.set((size_t)(j - 1) * C + col, N);
       const double2 rhs = d2(uc.x + dt * r.x, uc.y + dt * r.y);
       const double a1 = -beta * D2U(1, j), a2 = 1.0 - beta * (D2U(2, j) + m.k2x + m.k2z), a3 = -beta * D2U(3, j);
       w2 = 1.0 / (a2 - a1 * w3);
@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
         double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, cur.nu)), rmul(C3, csub(cur.pn, pc)));
         const double2 N2 = cadd(cur.nx, rmul(a.dthdyv[j], csub(cur.vvn, vvc_c)));
         r2 = cadd(r2, rmul(Cg, N2));
-        a.Nu2[(size_t)(j - 1) * C + col] = N2;
+        a.Nu2.set((size_t)(j - 1) * C + col, N2);
         const double2 rhs2 = d2(u2c.x + dt * r2.x, u2c.y + dt * r2.y);
         const double a1 = -beta * D2V(1, j), a2 = 1.0 - beta * (D2V(2, j) + m.k2x + m.k2z), a3 = -beta * D2V(3, j);
         w2 = 1.0 / (a2 - a1 * w3v);
@@ -897,6 +897,21 @@ void launch_fused_final(const ColArgs &a, int variant, hipStream_t st) {
   else if (variant == 2) fused_final4_kernel<8><<<nblk(a.C, 64), 256, 0, st>>>(a);
   else fused_final_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a);
 }
+
+__global__ void widen_kernel(const float2 *in, double2 *out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = d2(in[i].x, in[i].y);
+}
+__global__ void narrow_kernel(const double2 *in, float2 *out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float2 v;
+    v.x = (float)in[i].x;
+    v.y = (float)in[i].y;
+    out[i] = v;
+  }
+}
+void launch_widen(const float2 *in, double2 *out, size_t n, hipStream_t st) { widen_kernel<<<1024, 256, 0, st>>>(in, out, n); }
+void launch_narrow(const double2 *in, float2 *out, size_t n, hipStream_t st) { narrow_kernel<<<1024, 256, 0, st>>>(in, out, n); }
 
 void launch_ref_to_dev(const double2 *ref, double2 *dev, int C, int nrows, hipStream_t st) {
   // ref is [C][nrows] (rows_in = C, cols_in = nrows) -> dev [nrows][C]

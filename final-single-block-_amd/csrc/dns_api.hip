@@ -58,6 +58,7 @@ struct dns_handle {
   void *I, *P, *twz, *twx, *twh;
   double2 *twz64, *twx64, *twh64;  // FP64 twiddles for the reference-format plane utilities
   double2 *stage;                  // layout conversion buffer
+  double2 *stage2 = nullptr;       // FP64 copy of an FP32 array (mixed mode)
   DnsState *st;
   double *cfls = nullptr;  // CFL maxima spread over slots (plane stage at kRK = 1)
   std::vector<void *> allocs;
@@ -243,6 +244,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   rc |= chk(dalloc(h, &h->d_dthdyu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_dthdyv, (size_t)ny));
   rc |= chk(dalloc(h, &h->d_dyu2i, 3 * ((size_t)ny + 1))); rc |= chk(dalloc(h, &h->d_dyv2i, 3 * (size_t)ny));
   rc |= chk(dalloc(h, &h->stage, C * ru));
+  if (h->fp32) rc |= chk(dalloc(h, &h->stage2, C * ru));
   rc |= chk(dalloc(h, &h->st, 1));
   const size_t rsz = h->fp32 ? sizeof(float2) : sizeof(double2);
   const int Kp = h->fp32 ? dnsk::row_pitch<float>(h->Kx) : dnsk::row_pitch<double>(h->Kx);
@@ -364,8 +366,11 @@ static ColArgs col_args(dns_handle *h, int kRK) {
   a.yu = h->d_yu; a.yv = h->d_yv; a.dthdyu = h->d_dthdyu; a.dthdyv = h->d_dthdyv;
   a.dyu2i = h->d_dyu2i; a.dyv2i = h->d_dyv2i;
   a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.psi = h->psi; a.div = h->div;
-  a.du1 = h->du1; a.du2 = h->du2; a.du3 = h->du3; a.Nu1 = h->Nu1; a.Nu2 = h->Nu2; a.Nu3 = h->Nu3;
-  a.Nxz1 = h->Nxz1; a.Nxz2 = h->Nxz2; a.Nxz3 = h->Nxz3; a.uvf = h->uvf; a.vvc = h->vvc; a.wvf = h->wvf;
+  a.du1 = h->du1; a.du2 = h->du2; a.du3 = h->du3;
+  const int f = h->fp32 ? 1 : 0;  // nonlinear terms and fluxes are stored in the plane precision
+  a.Nu1 = {h->Nu1, f}; a.Nu2 = {h->Nu2, f}; a.Nu3 = {h->Nu3, f};
+  a.Nxz1 = {h->Nxz1, f}; a.Nxz2 = {h->Nxz2, f}; a.Nxz3 = {h->Nxz3, f};
+  a.uvf = {h->uvf, f}; a.vvc = {h->vvc, f}; a.wvf = {h->wvf, f};
   a.w3u = h->w3u; a.w3v = h->w3v; a.cP = h->cP; a.mscr = h->mscr;
   a.st = h->st;
   return a;
@@ -458,6 +463,13 @@ extern "C" int dns_upload_fields(dns_handle *h, const double *u1, const double *
   return DNS_OK;
 }
 
+// nonlinear terms and fluxes are FP32 in mixed mode: widened into stage2 before the layout conversion
+static const double2 *wide(dns_handle *h, const double2 *dev, int rows) {
+  if (!h->fp32) return dev;
+  dnsk::launch_widen(reinterpret_cast<const float2 *>(dev), h->stage2, (size_t)h->C * rows, h->stream);
+  return h->stage2;
+}
+
 static int download_dev(dns_handle *h, const double2 *dev, double *dst, int rows) {
   const size_t C = h->C;
   dnsk::launch_dev_to_ref(dev, h->stage, (int)C, rows, h->stream);
@@ -487,18 +499,25 @@ extern "C" int dns_download_fields(dns_handle *h, double *u1, double *u2, double
 extern "C" int dns_download_nonlinear(dns_handle *h, double *Nu1, double *Nu2, double *Nu3) {
   if (!h) return fail(DNS_ERR_ARG, "dns_download_nonlinear: null handle");
   int rc = 0;
-  if (Nu1 && (rc = download_dev(h, h->Nu1, Nu1, h->nyu))) return rc;
-  if (Nu2 && (rc = download_dev(h, h->Nu2, Nu2, h->nyv))) return rc;
-  if (Nu3 && (rc = download_dev(h, h->Nu3, Nu3, h->nyu))) return rc;
+  if (Nu1 && (rc = download_dev(h, wide(h, h->Nu1, h->nyu), Nu1, h->nyu))) return rc;
+  if (Nu2 && (rc = download_dev(h, wide(h, h->Nu2, h->nyv), Nu2, h->nyv))) return rc;
+  if (Nu3 && (rc = download_dev(h, wide(h, h->Nu3, h->nyu), Nu3, h->nyu))) return rc;
   return DNS_OK;
 }
 
 extern "C" int dns_upload_nonlinear(dns_handle *h, const double *Nu1, const double *Nu2, const double *Nu3) {
   if (!h || !Nu1 || !Nu2 || !Nu3) return fail(DNS_ERR_ARG, "dns_upload_nonlinear: null argument");
   int rc = 0;
-  if ((rc = upload_dev(h, h->Nu1, Nu1, h->nyu))) return rc;
-  if ((rc = upload_dev(h, h->Nu2, Nu2, h->nyv))) return rc;
-  if ((rc = upload_dev(h, h->Nu3, Nu3, h->nyu))) return rc;
+  double2 *dst[3] = {h->Nu1, h->Nu2, h->Nu3};
+  const double *srcs[3] = {Nu1, Nu2, Nu3};
+  const int rows[3] = {h->nyu, h->nyv, h->nyu};
+  for (int k = 0; k < 3; k++) {
+    if ((rc = upload_dev(h, h->fp32 ? h->stage2 : dst[k], srcs[k], rows[k]))) return rc;
+    if (h->fp32) {
+      dnsk::launch_narrow(h->stage2, reinterpret_cast<float2 *>(dst[k]), (size_t)h->C * rows[k], h->stream);
+      HIPCHK(hipStreamSynchronize(h->stream));
+    }
+  }
   return DNS_OK;
 }
 
@@ -515,9 +534,9 @@ extern "C" int dns_download_rhs(dns_handle *h, double *du1, double *du2, double 
 extern "C" int dns_download_flux(dns_handle *h, double *uv_f, double *vv_c, double *wv_f) {
   if (!h) return fail(DNS_ERR_ARG, "dns_download_flux: null handle");
   int rc = 0;
-  if (uv_f && (rc = download_dev(h, h->uvf, uv_f, h->nyv + 2))) return rc;
-  if (vv_c && (rc = download_dev(h, h->vvc, vv_c, h->nyu + 2))) return rc;
-  if (wv_f && (rc = download_dev(h, h->wvf, wv_f, h->nyv + 2))) return rc;
+  if (uv_f && (rc = download_dev(h, wide(h, h->uvf, h->nyv + 2), uv_f, h->nyv + 2))) return rc;
+  if (vv_c && (rc = download_dev(h, wide(h, h->vvc, h->nyu + 2), vv_c, h->nyu + 2))) return rc;
+  if (wv_f && (rc = download_dev(h, wide(h, h->wvf, h->nyv + 2), wv_f, h->nyv + 2))) return rc;
   return DNS_OK;
 }
 

@@ -28,7 +28,7 @@ struct PlaneArgs {
   const double *yu, *yv;
   void *I, *P;                     // plane intermediates, precision of the plane stage
   const void *twz, *twx, *twh;     // twiddles e^{-2 pi i m/N} for N = Mz, Mx, Mx/2
-  double2 *Nxz1, *Nxz2, *Nxz3, *uvf, *vvc, *wvf;
+  void *Nxz1, *Nxz2, *Nxz3, *uvf, *vvc, *wvf;  // complex of the plane precision
   double *cfl;                     // CFL_SLOTS partial max|u1|, then CFL_SLOTS partial max|u3|
 };
 
@@ -41,6 +41,33 @@ struct DnsState {
   double pad;
 };
 
+// Complex array of nonlinear terms or fluxes: FP64, or FP32 storage in mixed precision (f32 is uniform,
+// so the test costs a scalar branch).
+struct NArr {
+  void *p;
+  int f32;
+  __device__ __forceinline__ double2 operator[](size_t i) const {
+    if (f32) {
+      const float2 v = static_cast<const float2 *>(p)[i];
+      double2 r;
+      r.x = v.x;
+      r.y = v.y;
+      return r;
+    }
+    return static_cast<const double2 *>(p)[i];
+  }
+  __device__ __forceinline__ void set(size_t i, double2 v) const {
+    if (f32) {
+      float2 w;
+      w.x = (float)v.x;
+      w.y = (float)v.y;
+      static_cast<float2 *>(p)[i] = w;
+    } else {
+      static_cast<double2 *>(p)[i] = v;
+    }
+  }
+};
+
 struct ColArgs {
   int Kx, Nx, Nz, C, nyu, nyv, kRK, flag_ctpress;
   int fixed;                       // 1: dt is fixed (no CFL update)
@@ -50,8 +77,8 @@ struct ColArgs {
   const double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i;
   double2 *u1, *u2, *u3, *p, *psi, *div;
   double2 *du1, *du2, *du3;
-  double2 *Nu1, *Nu2, *Nu3;
-  const double2 *Nxz1, *Nxz2, *Nxz3, *uvf, *vvc, *wvf;
+  NArr Nu1, Nu2, Nu3;              // nonlinear terms of the substep (old ones on entry)
+  NArr Nxz1, Nxz2, Nxz3, uvf, vvc, wvf;  // plane-stage outputs
   double *w3u, *w3v, *cP;          // per-row scratch of the eliminations
   double *mscr;                    // 2*(nyu+2) doubles for the mean-mode solve
   DnsState *st;
@@ -63,6 +90,8 @@ int plane_variants(int n);          // tile variants instantiated for transform 
 // which: 0 z-inverse (nw = nbj + 1 field rows), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes (nw = nbj)
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);
 void launch_cfl_reduce(const double *slots, double *out, hipStream_t st);
+void launch_widen(const float2 *in, double2 *out, size_t n, hipStream_t st);
+void launch_narrow(const double2 *in, float2 *out, size_t n, hipStream_t st);
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st);
 void launch_zero(double *p, int n, hipStream_t st);
 int launch_plane_roundtrip(int Mx, int Mz, const double *in, double *phys, double *back, int n,

@@ -209,6 +209,8 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
   const cx<R> *P = reinterpret_cast<const cx<R> *>(a.P) + (size_t)((VPL ? a.nbj : 0) + jj) * 4 * MZ * Kp;
   const double inv = 1.0 / (double)MZ;
   const size_t ju = (size_t)(j - 1) * C, jf = (size_t)j * C;
+  // outputs are stored in the plane precision (FP32 in mixed mode)
+  auto put = [](void *base, size_t i, double2 v) { reinterpret_cast<cx<R> *>(base)[i] = from_d2<R>(v); };
   double2 accA[PERMAX], accB[PERMAX];
 #pragma unroll
   for (int step = 0; step < NP; step++) {
@@ -235,15 +237,15 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
         const size_t col = (size_t)k * Kx + i;
         if (!VPL) {
           if (step == 0) { accA[q] = imul(kz, v); accB[q] = imul(kx, v); }
-          else if (step == 1) { if (st) a.Nxz1[ju + col] = masked ? d2(0.0, 0.0) : cadd(imul(kx, v), accA[q]); }
-          else if (step == 2) { if (st) a.Nxz3[ju + col] = masked ? d2(0.0, 0.0) : cadd(accB[q], imul(kz, v)); }
-          else { if (st) a.vvc[jf + col] = v; }
+          else if (step == 1) { if (st) put(a.Nxz1, ju + col, masked ? d2(0.0, 0.0) : cadd(imul(kx, v), accA[q])); }
+          else if (step == 2) { if (st) put(a.Nxz3, ju + col, masked ? d2(0.0, 0.0) : cadd(accB[q], imul(kz, v))); }
+          else { if (st) put(a.vvc, jf + col, v); }
         } else {
-          if (step == 0) { accA[q] = imul(kx, v); if (st) a.uvf[jf + col] = v; }
+          if (step == 0) { accA[q] = imul(kx, v); if (st) put(a.uvf, jf + col, v); }
           else {
             if (st) {
-              if (j <= a.nyv) a.Nxz2[ju + col] = masked ? d2(0.0, 0.0) : cadd(accA[q], imul(kz, v));
-              a.wvf[jf + col] = v;
+              if (j <= a.nyv) put(a.Nxz2, ju + col, masked ? d2(0.0, 0.0) : cadd(accA[q], imul(kz, v)));
+              put(a.wvf, jf + col, v);
             }
           }
         }
