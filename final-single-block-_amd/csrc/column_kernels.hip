@@ -73,7 +73,8 @@ __device__ __forceinline__ double substep_dt(const ColArgs &a, int col) {
 
 // ================================================================================================
 // staged path
-__global__ void rhs0_kernel(ColArgs a) {
+__global__ void rhs0_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const int C = a.C;
@@ -104,7 +105,8 @@ __global__ void rhs0_kernel(ColArgs a) {
   }
 }
 
-__global__ void nl_col_kernel(ColArgs a) {
+__global__ void nl_col_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const int C = a.C;
@@ -173,7 +175,8 @@ __device__ __forceinline__ void solve_column(const ColArgs &a, bool ugrid, doubl
   }
 }
 
-__global__ void solveU_kernel(ColArgs a) {
+__global__ void solveU_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const Mode m = col_mode(a, col);
@@ -186,7 +189,8 @@ __global__ void solveU_kernel(ColArgs a) {
 
 // flowrate_corr (littleharsh.f90:515-548) with LUsol0/LU_build0 (tridLU_3D.f90:134-160, 299-325);
 // one thread: mode (0,0) is column 0.
-__global__ void meanflow_kernel(ColArgs a) {
+__global__ void meanflow_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const int n = a.nyu + 2, C = a.C;
   const double dt = a.st->dt;
@@ -228,7 +232,8 @@ __global__ void meanflow_kernel(ColArgs a) {
 }
 
 // flowrateIm of mode (0,0) : littleharsh.f90:473-492, meanflow_ctP :572-589
-__global__ void qx_kernel(ColArgs a) {
+__global__ void qx_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double Q = 0.0;
   for (int j = 1; j <= a.nyv + 1; j++) Q = Q + (a.yv[j] - a.yv[j - 1]) * a.u1[(size_t)j * a.C].x;
@@ -260,7 +265,8 @@ __device__ __forceinline__ void dg_row(const ColArgs &a, int col, double k2x, do
   }
 }
 
-__global__ void solveP_kernel(ColArgs a) {
+__global__ void solveP_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const int C = a.C, e = a.nyu;
@@ -301,7 +307,8 @@ __global__ void solveP_kernel(ColArgs a) {
 }
 
 template <int NTH>
-__global__ void __launch_bounds__(NTH) vcorr_kernel(ColArgs a) {
+__global__ void __launch_bounds__(NTH) vcorr_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   __shared__ double red[NTH / 64];
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   double dmax = 0.0;
@@ -338,7 +345,8 @@ __global__ void __launch_bounds__(NTH) vcorr_kernel(ColArgs a) {
 
 // ================================================================================================
 // fused path
-__global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a) {
+__global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const int C = a.C, nyu = a.nyu, nyv = a.nyv;
@@ -438,7 +446,8 @@ __global__ void __launch_bounds__(256) fused_up_kernel(ColArgs a) {
 // each one streams a third of the rows, so more memory requests are in flight.  With PF the next row's
 // inputs are loaded before the current row is computed.  Arithmetic identical to fused_up_kernel.
 template <bool PF>
-__global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
+__global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int f = threadIdx.x >> 6;
   const int col = blockIdx.x * 64 + (threadIdx.x & 63);
   if (col >= a.C) return;
@@ -555,7 +564,8 @@ __global__ void __launch_bounds__(192) fused_up3_kernel(ColArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256) fused_down_kernel(ColArgs a) {
+__global__ void __launch_bounds__(256) fused_down_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= a.C) return;
   const int C = a.C, nyu = a.nyu;
@@ -602,7 +612,8 @@ __global__ void __launch_bounds__(256) fused_down_kernel(ColArgs a) {
 // u2 and hand each chunk of K rows to wave 3 through double-buffered LDS; wave 3 forms the divergence and
 // eliminates the pressure system one chunk behind.  Arithmetic identical to fused_down_kernel.
 template <int K>
-__global__ void __launch_bounds__(256) fused_down4_kernel(ColArgs a) {
+__global__ void __launch_bounds__(256) fused_down4_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   __shared__ double2 buf[2][3][K][64];
   const int f = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = blockIdx.x * 64 + lane;
@@ -665,7 +676,8 @@ __global__ void __launch_bounds__(256) fused_down4_kernel(ColArgs a) {
 // forms the divergence of the projected velocities one further chunk behind.  Arithmetic identical to
 // fused_final_kernel.
 template <int K>
-__global__ void __launch_bounds__(256) fused_final4_kernel(ColArgs a) {
+__global__ void __launch_bounds__(256) fused_final4_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   __shared__ double2 psb[2][K][64];     // ps of rows j (chunk parity)
   __shared__ double2 vb[2][3][K][64];   // projected u1(j), u3(j), v(j-1)
   __shared__ double red[4];
@@ -768,7 +780,8 @@ __global__ void __launch_bounds__(256) fused_final4_kernel(ColArgs a) {
 }
 
 template <int NTH>
-__global__ void __launch_bounds__(NTH) fused_final_kernel(ColArgs a) {
+__global__ void __launch_bounds__(NTH) fused_final_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   __shared__ double red[NTH / 64];
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   double dmax = 0.0;
@@ -875,27 +888,27 @@ __global__ void planes_to_modes_kernel(const double *planes, double2 *cols, int 
 // ================================================================================================
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-void launch_rhs0(const ColArgs &a, hipStream_t st) { rhs0_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_nl_col(const ColArgs &a, hipStream_t st) { nl_col_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_solveU(const ColArgs &a, hipStream_t st) { solveU_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_meanflow(const ColArgs &a, hipStream_t st) { meanflow_kernel<<<1, 64, 0, st>>>(a); }
-void launch_qx(const ColArgs &a, hipStream_t st) { qx_kernel<<<1, 64, 0, st>>>(a); }
-void launch_solveP(const ColArgs &a, hipStream_t st) { solveP_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a); }
-void launch_vcorr(const ColArgs &a, hipStream_t st) { vcorr_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a); }
+void launch_rhs0(const ColArgs &a, hipStream_t st) { rhs0_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a); }
+void launch_nl_col(const ColArgs &a, hipStream_t st) { nl_col_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a); }
+void launch_solveU(const ColArgs &a, hipStream_t st) { solveU_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a); }
+void launch_meanflow(const ColArgs &a, hipStream_t st) { meanflow_kernel<<<dim3(1, a.nm), 64, 0, st>>>(a); }
+void launch_qx(const ColArgs &a, hipStream_t st) { qx_kernel<<<dim3(1, a.nm), 64, 0, st>>>(a); }
+void launch_solveP(const ColArgs &a, hipStream_t st) { solveP_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a); }
+void launch_vcorr(const ColArgs &a, hipStream_t st) { vcorr_kernel<256><<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a); }
 void launch_fused_up(const ColArgs &a, int variant, hipStream_t st) {
-  if (variant == 1) fused_up3_kernel<false><<<nblk(a.C, 64), 192, 0, st>>>(a);
-  else if (variant == 2) fused_up3_kernel<true><<<nblk(a.C, 64), 192, 0, st>>>(a);
-  else fused_up_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a);
+  if (variant == 1) fused_up3_kernel<false><<<dim3(nblk(a.C, 64), a.nm), 192, 0, st>>>(a);
+  else if (variant == 2) fused_up3_kernel<true><<<dim3(nblk(a.C, 64), a.nm), 192, 0, st>>>(a);
+  else fused_up_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
 }
 void launch_fused_down(const ColArgs &a, int variant, hipStream_t st) {
-  if (variant == 1) fused_down4_kernel<4><<<nblk(a.C, 64), 256, 0, st>>>(a);
-  else if (variant == 2) fused_down4_kernel<8><<<nblk(a.C, 64), 256, 0, st>>>(a);
-  else fused_down_kernel<<<nblk(a.C, 256), 256, 0, st>>>(a);
+  if (variant == 1) fused_down4_kernel<4><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
+  else if (variant == 2) fused_down4_kernel<8><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
+  else fused_down_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
 }
 void launch_fused_final(const ColArgs &a, int variant, hipStream_t st) {
-  if (variant == 1) fused_final4_kernel<4><<<nblk(a.C, 64), 256, 0, st>>>(a);
-  else if (variant == 2) fused_final4_kernel<8><<<nblk(a.C, 64), 256, 0, st>>>(a);
-  else fused_final_kernel<256><<<nblk(a.C, 256), 256, 0, st>>>(a);
+  if (variant == 1) fused_final4_kernel<4><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
+  else if (variant == 2) fused_final4_kernel<8><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
+  else fused_final_kernel<256><<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
 }
 
 __global__ void widen_kernel(const float2 *in, double2 *out, size_t n) {

@@ -45,7 +45,11 @@ struct dns_handle {
   int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0};
   size_t bytes = 0;
   hipStream_t stream = nullptr;
-  bool uploaded = false, timing = false;
+  bool timing = false;
+  int nm = 1;                      // ensemble members (same sizes, own Re, mpgx, state)
+  std::vector<char> uploaded;      // per member
+  std::vector<double> Re_m, mpgx_m;
+  double dtv_den = 1.0;            // dtv = Re / dtv_den
   // host set-up (start.f90)
   std::vector<double> yu, yv, dthdyu, dthdyv, dyu2i, dyv2i;
   double dtv, gw1, gw2, gbc, gi1, gi2;
@@ -139,7 +143,8 @@ static void host_grid(dns_handle *h) {
   // dtv (start.f90:476) with k2F_x(Nx/2) and k2F_z(Nz/2) of def_k (start.f90:838-862)
   const double k2x = -((h->prm.alp * (h->Nx / 2)) * (h->prm.alp * (h->Nx / 2)));
   const double k2z = -((h->prm.bet * (h->Nz / 2 - 1)) * (h->prm.bet * (h->Nz / 2 - 1)));
-  h->dtv = h->prm.Re / (-k2x - k2z + 4.0 / sq(yv[1] - yv[0]));
+  h->dtv_den = -k2x - k2z + 4.0 / sq(yv[1] - yv[0]);
+  h->dtv = h->prm.Re / h->dtv_den;
   // wall weights (start.f90:518-537)
   const int nyu = h->nyu, nyv = h->nyv;
   h->gw1 = -(yv[0] - yu[0]) / (yv[0] - yu[1]);
@@ -224,13 +229,18 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   h->Kx = h->Nx / 2 + 1; h->C = h->Kx * h->Nz;
   h->nyv = ny - 2; h->nyu = h->nyv + 1;
   h->fp32 = prm->precision == DNS_PREC_MIXED;
+  h->nm = prm->members > 1 ? prm->members : 1;
+  h->uploaded.assign(h->nm, 0);
+  h->Re_m.assign(h->nm, prm->Re);
+  h->mpgx_m.assign(h->nm, prm->mpgx);
   const int nplanes = 2 * h->nyu;
   h->bj = (prm->plane_batch > 0 && prm->plane_batch < nplanes) ? (prm->plane_batch + 1) / 2 : h->nyu;
   host_grid(h);
 
   auto chk = [&](int rc) { return rc; };
   int rc = 0;
-  const size_t C = h->C, ru = h->nyu + 2, rv = h->nyv + 2, rp = h->nyu;
+  const size_t C = h->C, ru = (size_t)(h->nyu + 2) * h->nm, rv = (size_t)(h->nyv + 2) * h->nm,
+               rp = (size_t)h->nyu * h->nm;  // rows of all members
   rc |= chk(dalloc(h, &h->u1, C * ru)); rc |= chk(dalloc(h, &h->u3, C * ru)); rc |= chk(dalloc(h, &h->u2, C * rv));
   rc |= chk(dalloc(h, &h->p, C * rp)); rc |= chk(dalloc(h, &h->psi, C * rp)); rc |= chk(dalloc(h, &h->div, C * rp));
   rc |= chk(dalloc(h, &h->du1, C * ru)); rc |= chk(dalloc(h, &h->du3, C * ru)); rc |= chk(dalloc(h, &h->du2, C * rv));
@@ -239,16 +249,16 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   rc |= chk(dalloc(h, &h->uvf, C * rv)); rc |= chk(dalloc(h, &h->wvf, C * rv)); rc |= chk(dalloc(h, &h->vvc, C * ru));
   rc |= chk(dalloc(h, &h->w3u, C * ru)); rc |= chk(dalloc(h, &h->w3v, C * rv)); rc |= chk(dalloc(h, &h->cP, C * rp));
   rc |= chk(dalloc(h, &h->mscr, 2 * ru));
-  rc |= chk(dalloc(h, &h->cfls, 2 * dnsk::CFL_SLOTS));
+  rc |= chk(dalloc(h, &h->cfls, 2 * dnsk::CFL_SLOTS * (size_t)h->nm));
   rc |= chk(dalloc(h, &h->d_yu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_yv, (size_t)ny));
   rc |= chk(dalloc(h, &h->d_dthdyu, (size_t)ny + 1)); rc |= chk(dalloc(h, &h->d_dthdyv, (size_t)ny));
   rc |= chk(dalloc(h, &h->d_dyu2i, 3 * ((size_t)ny + 1))); rc |= chk(dalloc(h, &h->d_dyv2i, 3 * (size_t)ny));
-  rc |= chk(dalloc(h, &h->stage, C * ru));
-  if (h->fp32) rc |= chk(dalloc(h, &h->stage2, C * ru));
-  rc |= chk(dalloc(h, &h->st, 1));
+  rc |= chk(dalloc(h, &h->stage, C * (h->nyu + 2)));  // one member's field
+  if (h->fp32) rc |= chk(dalloc(h, &h->stage2, C * (h->nyu + 2)));
+  rc |= chk(dalloc(h, &h->st, (size_t)h->nm));
   const size_t rsz = h->fp32 ? sizeof(float2) : sizeof(double2);
   const int Kp = h->fp32 ? dnsk::row_pitch<float>(h->Kx) : dnsk::row_pitch<double>(h->Kx);
-  const size_t iel = (size_t)2 * h->bj * 3 * Mz * Kp, pel = (size_t)2 * h->bj * 4 * Mz * Kp;
+  const size_t iel = dnsk::member_istride(h->bj, Mz, Kp) * h->nm, pel = dnsk::member_pstride(h->bj, Mz, Kp) * h->nm;
   char *Ib = nullptr, *Pb = nullptr, *tz = nullptr, *tx = nullptr, *th = nullptr;
   rc |= chk(dalloc(h, &Ib, iel * rsz)); rc |= chk(dalloc(h, &Pb, pel * rsz));
   rc |= chk(dalloc(h, &tz, (size_t)Mz * rsz)); rc |= chk(dalloc(h, &tx, (size_t)Mx * rsz));
@@ -286,10 +296,14 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
     e = e ? e : up(h->twx64, b.data(), b.size() * 8);
     e = e ? e : up(h->twh64, c.data(), c.size() * 8);
   }
-  DnsState s0;
-  memset(&s0, 0, sizeof s0);
-  s0.mpgx = prm->mpgx;
-  e = e ? e : up(h->st, &s0, sizeof s0);
+  for (int m = 0; m < h->nm; m++) {
+    DnsState s0;
+    memset(&s0, 0, sizeof s0);
+    s0.mpgx = prm->mpgx;
+    s0.Re = prm->Re;
+    s0.dtv = h->dtv;
+    e = e ? e : up(h->st + m, &s0, sizeof s0);
+  }
   e = e ? e : hipStreamCreate(&h->stream);
   for (int q = 0; q < 8 && e == hipSuccess; q++) e = hipEventCreate(&h->ev[q]);
   if (e != hipSuccess) {
@@ -357,6 +371,7 @@ static ColArgs col_args(dns_handle *h, int kRK) {
   memset(&a, 0, sizeof a);
   a.Kx = h->Kx; a.Nx = h->Nx; a.Nz = h->Nz; a.C = h->C; a.nyu = h->nyu; a.nyv = h->nyv;
   a.kRK = kRK; a.flag_ctpress = h->prm.flag_ctpress;
+  a.nm = h->nm;
   a.fixed = h->prm.fixed_dt > 0.0; a.fixed_dt = h->prm.fixed_dt;
   a.Re = h->prm.Re; a.alp = h->prm.alp; a.bet = h->prm.bet; a.CFL = h->prm.CFL; a.dtv = h->dtv;
   const int q = kRK - 1;
@@ -381,6 +396,7 @@ static PlaneArgs plane_args(dns_handle *h, int kRK) {
   memset(&a, 0, sizeof a);
   a.Mx = h->Mx; a.Mz = h->Mz; a.Nx = h->Nx; a.Nz = h->Nz; a.Kx = h->Kx; a.C = h->C;
   a.nyu = h->nyu; a.nyv = h->nyv; a.kRK = kRK;
+  a.nm = h->nm; a.bjcap = h->bj;
   a.alp = h->prm.alp; a.bet = h->prm.bet;
   a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.yu = h->d_yu; a.yv = h->d_yv;
   a.I = h->I; a.P = h->P; a.twz = h->twz; a.twx = h->twx; a.twh = h->twh;
@@ -392,7 +408,8 @@ static PlaneArgs plane_args(dns_handle *h, int kRK) {
 static int check_kRK(dns_handle *h, int kRK, const char *who) {
   if (!h) return fail(DNS_ERR_ARG, "%s: null handle", who);
   if (kRK < 1 || kRK > 3) return fail(DNS_ERR_ARG, "%s: kRK=%d not in 1..3", who, kRK);
-  if (!h->uploaded) return fail(DNS_ERR_STATE, "%s: no fields uploaded", who);
+  for (int m = 0; m < h->nm; m++)
+    if (!h->uploaded[m]) return fail(DNS_ERR_STATE, "%s: no fields uploaded for member %d", who, m);
   return DNS_OK;
 }
 
@@ -403,9 +420,9 @@ static void tmark(dns_handle *h, int q) {
 // nonlinear plane stage + CFL maxima (FOU3D.f90:98-303 up to the column part)
 static int plane_stage(dns_handle *h, int kRK) {
   if (kRK == 1) {
-    dnsk::launch_zero(&h->st->cfl[0], 3, h->stream);  // cfl[0], cfl[1], divmax
-    dnsk::launch_zero(h->cfls, 2 * dnsk::CFL_SLOTS, h->stream);
-    dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, h->cfls, h->stream);
+    dnsk::launch_cfl_reset(h->st, h->cfls, h->nm, h->stream);  // cfl[0], cfl[1], divmax, slots
+    dnsk::launch_cfl_ghost(h->u1, h->u3, h->C, dnsk::member_strides(h->C, h->nyu, h->nyv).ru, h->nm, h->cfls,
+                           h->stream);
   }
   PlaneArgs a = plane_args(h, kRK);
   const bool one_group = h->bj >= h->nyu;  // per-kernel event marks only with one launch per kernel
@@ -428,18 +445,21 @@ static int plane_stage(dns_handle *h, int kRK) {
     tmark(h, 2);
     tmark(h, 3);
   }
-  if (kRK == 1) dnsk::launch_cfl_reduce(h->cfls, &h->st->cfl[0], h->stream);
+  if (kRK == 1) dnsk::launch_cfl_reduce(h->cfls, h->st, h->nm, h->stream);
   HIPCHK(hipGetLastError());
   return DNS_OK;
 }
 
 // ------------------------------------------------------------------------------------------------
-extern "C" int dns_upload_fields(dns_handle *h, const double *u1, const double *u2, const double *u3,
+extern "C" int dns_upload_member(dns_handle *h, int m, const double *u1, const double *u2, const double *u3,
                                  const double *p, double t, int iter, double QxT) {
-  if (!h || !u1 || !u2 || !u3 || !p) return fail(DNS_ERR_ARG, "dns_upload_fields: null argument");
+  if (!h || !u1 || !u2 || !u3 || !p) return fail(DNS_ERR_ARG, "dns_upload_member: null argument");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_upload_member: member %d not in 0..%d", m, h->nm - 1);
   const size_t C = h->C;
+  const dnsk::MemberStrides ms = dnsk::member_strides(h->C, h->nyu, h->nyv);
   struct F { const double *src; double2 *dst; int rows; } f[4] = {
-      {u1, h->u1, h->nyu + 2}, {u2, h->u2, h->nyv + 2}, {u3, h->u3, h->nyu + 2}, {p, h->p, h->nyu}};
+      {u1, h->u1 + m * ms.ru, h->nyu + 2}, {u2, h->u2 + m * ms.rv, h->nyv + 2},
+      {u3, h->u3 + m * ms.ru, h->nyu + 2}, {p, h->p + m * ms.rp, h->nyu}};
   for (auto &x : f) {
     HIPCHK(hipMemcpyAsync(h->stage, x.src, C * x.rows * sizeof(double2), hipMemcpyHostToDevice, h->stream));
     dnsk::launch_ref_to_dev(h->stage, x.dst, (int)C, x.rows, h->stream);
@@ -451,15 +471,45 @@ extern "C" int dns_upload_fields(dns_handle *h, const double *u1, const double *
   memset(&s0, 0, sizeof s0);
   s0.t = t;
   s0.iter = iter;
-  s0.mpgx = h->prm.mpgx;
+  s0.mpgx = h->mpgx_m[m];
   s0.Qx = Qx;
   s0.QxT = QxT > 0.0 ? QxT : Qx;
-  HIPCHK(hipMemcpyAsync(h->st, &s0, sizeof s0, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemsetAsync(h->Nu1, 0, C * h->nyu * sizeof(double2), h->stream));
-  HIPCHK(hipMemsetAsync(h->Nu2, 0, C * h->nyu * sizeof(double2), h->stream));
-  HIPCHK(hipMemsetAsync(h->Nu3, 0, C * h->nyu * sizeof(double2), h->stream));
+  s0.Re = h->Re_m[m];
+  s0.dtv = h->Re_m[m] / h->dtv_den;
+  HIPCHK(hipMemcpyAsync(h->st + m, &s0, sizeof s0, hipMemcpyHostToDevice, h->stream));
+  // the carried nonlinear terms start at zero (their weight at kRK = 1 is cRK = 0)
+  const size_t nb = ms.rp * (h->fp32 ? sizeof(float2) : sizeof(double2));
+  HIPCHK(hipMemsetAsync(reinterpret_cast<char *>(h->Nu1) + m * nb, 0, nb, h->stream));
+  HIPCHK(hipMemsetAsync(reinterpret_cast<char *>(h->Nu2) + m * nb, 0, nb, h->stream));
+  HIPCHK(hipMemsetAsync(reinterpret_cast<char *>(h->Nu3) + m * nb, 0, nb, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  h->uploaded = true;
+  h->uploaded[m] = 1;
+  return DNS_OK;
+}
+
+extern "C" int dns_upload_fields(dns_handle *h, const double *u1, const double *u2, const double *u3,
+                                 const double *p, double t, int iter, double QxT) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_upload_fields: null argument");
+  for (int m = 0; m < h->nm; m++) {
+    const int rc = dns_upload_member(h, m, u1, u2, u3, p, t, iter, QxT);
+    if (rc) return rc;
+  }
+  return DNS_OK;
+}
+
+extern "C" int dns_set_member_params(dns_handle *h, int m, double Re, double mpgx) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_set_member_params: null handle");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_set_member_params: member %d not in 0..%d", m, h->nm - 1);
+  if (!(Re > 0)) return fail(DNS_ERR_ARG, "dns_set_member_params: Re must be positive");
+  h->Re_m[m] = Re;
+  h->mpgx_m[m] = mpgx;
+  DnsState x;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&x, h->st + m, sizeof x, hipMemcpyDeviceToHost));
+  x.Re = Re;
+  x.dtv = Re / h->dtv_den;
+  x.mpgx = mpgx;
+  HIPCHK(hipMemcpy(h->st + m, &x, sizeof x, hipMemcpyHostToDevice));
   return DNS_OK;
 }
 
@@ -486,14 +536,20 @@ static int upload_dev(dns_handle *h, double2 *dev, const double *src, int rows) 
   return DNS_OK;
 }
 
-extern "C" int dns_download_fields(dns_handle *h, double *u1, double *u2, double *u3, double *p) {
-  if (!h) return fail(DNS_ERR_ARG, "dns_download_fields: null handle");
+extern "C" int dns_download_member(dns_handle *h, int m, double *u1, double *u2, double *u3, double *p) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_download_member: null handle");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_download_member: member %d not in 0..%d", m, h->nm - 1);
+  const dnsk::MemberStrides ms = dnsk::member_strides(h->C, h->nyu, h->nyv);
   int rc = 0;
-  if (u1 && (rc = download_dev(h, h->u1, u1, h->nyu + 2))) return rc;
-  if (u2 && (rc = download_dev(h, h->u2, u2, h->nyv + 2))) return rc;
-  if (u3 && (rc = download_dev(h, h->u3, u3, h->nyu + 2))) return rc;
-  if (p && (rc = download_dev(h, h->p, p, h->nyu))) return rc;
+  if (u1 && (rc = download_dev(h, h->u1 + m * ms.ru, u1, h->nyu + 2))) return rc;
+  if (u2 && (rc = download_dev(h, h->u2 + m * ms.rv, u2, h->nyv + 2))) return rc;
+  if (u3 && (rc = download_dev(h, h->u3 + m * ms.ru, u3, h->nyu + 2))) return rc;
+  if (p && (rc = download_dev(h, h->p + m * ms.rp, p, h->nyu))) return rc;
   return DNS_OK;
+}
+
+extern "C" int dns_download_fields(dns_handle *h, double *u1, double *u2, double *u3, double *p) {
+  return dns_download_member(h, 0, u1, u2, u3, p);
 }
 
 extern "C" int dns_download_nonlinear(dns_handle *h, double *Nu1, double *Nu2, double *Nu3) {
@@ -548,10 +604,11 @@ extern "C" int dns_download_pressure_correction(dns_handle *h, double *psi, doub
   return DNS_OK;
 }
 
-extern "C" int dns_get_scalars(dns_handle *h, double *s) {
-  if (!h || !s) return fail(DNS_ERR_ARG, "dns_get_scalars: null argument");
+extern "C" int dns_get_member_scalars(dns_handle *h, int m, double *s) {
+  if (!h || !s) return fail(DNS_ERR_ARG, "dns_get_member_scalars: null argument");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_get_member_scalars: member %d not in 0..%d", m, h->nm - 1);
   DnsState x;
-  HIPCHK(hipMemcpyAsync(&x, h->st, sizeof x, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&x, h->st + m, sizeof x, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   const double v[10] = {x.t, x.dt, x.dti, x.mpgx, x.Qx, x.dgx, x.dtc, x.QxT, x.iter, x.divmax};
   memcpy(s, v, sizeof v);
@@ -559,14 +616,18 @@ extern "C" int dns_get_scalars(dns_handle *h, double *s) {
   return DNS_OK;
 }
 
+extern "C" int dns_get_scalars(dns_handle *h, double *s) { return dns_get_member_scalars(h, 0, s); }
+
 extern "C" int dns_set_dt(dns_handle *h, double dt) {
   if (!h || !(dt > 0)) return fail(DNS_ERR_ARG, "dns_set_dt: bad argument");
   HIPCHK(hipStreamSynchronize(h->stream));
-  double v[3];
-  HIPCHK(hipMemcpy(v, h->st, 3 * sizeof(double), hipMemcpyDeviceToHost));
-  v[1] = dt;
-  v[2] = 1.0 / dt;
-  HIPCHK(hipMemcpy(h->st, v, 3 * sizeof(double), hipMemcpyHostToDevice));
+  for (int m = 0; m < h->nm; m++) {
+    double v[3];
+    HIPCHK(hipMemcpy(v, h->st + m, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    v[1] = dt;
+    v[2] = 1.0 / dt;
+    HIPCHK(hipMemcpy(h->st + m, v, 3 * sizeof(double), hipMemcpyHostToDevice));
+  }
   return DNS_OK;
 }
 
@@ -616,7 +677,7 @@ extern "C" int dns_solveP(dns_handle *h, int kRK) {
 extern "C" int dns_vcorr(dns_handle *h, int kRK) {
   int rc = check_kRK(h, kRK, "dns_vcorr");
   if (rc) return rc;
-  dnsk::launch_zero(&h->st->divmax, 1, h->stream);
+  for (int m = 0; m < h->nm; m++) dnsk::launch_zero(&h->st[m].divmax, 1, h->stream);
   dnsk::launch_vcorr(col_args(h, kRK), h->stream);
   HIPCHK(hipGetLastError());
   return DNS_OK;
@@ -736,7 +797,8 @@ extern "C" int dns_set_plane_variants(dns_handle *h, const int *var) {
 extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   if (!h) return fail(DNS_ERR_ARG, "dns_autotune: null handle");
   if (iters < 1 || iters > 1000) return fail(DNS_ERR_ARG, "dns_autotune: iters=%d not in 1..1000", iters);
-  if (!h->uploaded) return fail(DNS_ERR_STATE, "dns_autotune: no fields uploaded");
+  for (int m = 0; m < h->nm; m++)
+    if (!h->uploaded[m]) return fail(DNS_ERR_STATE, "dns_autotune: no fields uploaded for member %d", m);
   PlaneArgs a = plane_args(h, 2);
   a.j0 = 1;
   a.nbj = std::min(h->bj, h->nyu);
@@ -745,15 +807,16 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   HIPCHK(hipEventCreate(&e1));
   // the column sweeps overwrite u1, u2, u3, p and Nu: keep copies and put them back afterwards
   const size_t C = h->C;
-  const size_t nu = C * (h->nyu + 2), nv = C * (h->nyv + 2), nn = C * h->nyu;
+  const size_t nu = C * (h->nyu + 2) * h->nm, nv = C * (h->nyv + 2) * h->nm, nn = C * h->nyu * h->nm;
   double2 *save = nullptr;
   int rc = DNS_OK;
   const double2 *src[7] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3, h->p};
   const size_t len[7] = {nu, nu, nv, nn, nn, nn, nn};
-  DnsState st_save;
+  std::vector<DnsState> st_save(h->nm);
   {
     HIPCHK(hipMalloc(&save, (2 * nu + nv + 4 * nn) * sizeof(double2)));
-    HIPCHK(hipMemcpyAsync(&st_save, h->st, sizeof st_save, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(st_save.data(), h->st, h->nm * sizeof(DnsState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     double2 *q = save;
     for (int k = 0; k < 7; k++) {
       HIPCHK(hipMemcpyAsync(q, src[k], len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
@@ -798,7 +861,7 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
                            h->stream);
       q += len[k];
     }
-    (void)hipMemcpyAsync(h->st, &st_save, sizeof st_save, hipMemcpyHostToDevice, h->stream);
+    (void)hipMemcpyAsync(h->st, st_save.data(), h->nm * sizeof(DnsState), hipMemcpyHostToDevice, h->stream);
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(save);
   }

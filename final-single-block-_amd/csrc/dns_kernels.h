@@ -10,6 +10,8 @@ namespace dnsk {
 //   u1, u3, du1, du3, vvc : rows 0..nyu+1 at j*C        u2, du2, uvf, wvf : rows 0..nyv+1 at j*C
 //   p, psi, div, Nu1, Nu3, Nxz1, Nxz3 : rows 1..nyu at (j-1)*C
 //   Nu2, Nxz2 : rows 1..nyv at (j-1)*C
+// An ensemble of nm members of one size stores the members one after another in every array (member
+// strides in member_strides); kernels take the member from the last grid dimension.
 constexpr int CFL_SLOTS = 64;  // addresses the CFL maxima are spread over
 
 // Row pitch (complex elements of precision R) of the plane intermediates I and P: Kx rounded up to
@@ -23,6 +25,7 @@ struct PlaneArgs {
   int Mx, Mz, Nx, Nz, Kx, C, nyu, nyv;
   int j0, nbj;                     // planes j0..j0+nbj-1 (of both kinds) in this batch
   int kRK;
+  int nm, bjcap;                   // ensemble members; plane-index capacity of one member's I/P
   double alp, bet;
   const double2 *u1, *u2, *u3;
   const double *yu, *yv;
@@ -38,8 +41,23 @@ struct DnsState {
   double divmax;
   double iter;
   double nonfinite;                // set to 1 when a CFL maximum is NaN or Inf
-  double pad;
+  double Re, dtv;                  // this member's Reynolds number and viscous step limit
 };
+
+// Elements per member of the three row families (u-grid with ghosts, v-grid, p-grid).
+struct MemberStrides {
+  size_t ru, rv, rp;
+};
+__host__ __device__ __forceinline__ MemberStrides member_strides(int C, int nyu, int nyv) {
+  return {(size_t)(nyu + 2) * C, (size_t)(nyv + 2) * C, (size_t)nyu * C};
+}
+// plane-stage intermediates of one member (complex elements of the plane precision)
+__host__ __device__ __forceinline__ size_t member_istride(int bjcap, int Mz, int Kp) {
+  return (size_t)2 * bjcap * 3 * Mz * Kp;
+}
+__host__ __device__ __forceinline__ size_t member_pstride(int bjcap, int Mz, int Kp) {
+  return (size_t)2 * bjcap * 4 * Mz * Kp;
+}
 
 // Complex array of nonlinear terms or fluxes: FP64, or FP32 storage in mixed precision (f32 is uniform,
 // so the test costs a scalar branch).
@@ -70,6 +88,7 @@ struct NArr {
 
 struct ColArgs {
   int Kx, Nx, Nz, C, nyu, nyv, kRK, flag_ctpress;
+  int nm;                          // ensemble members (grid dimension y)
   int fixed;                       // 1: dt is fixed (no CFL update)
   double Re, alp, bet, CFL, dtv, fixed_dt;
   double aRK, bRK, gRK, cRK, dRK;
@@ -84,15 +103,58 @@ struct ColArgs {
   DnsState *st;
 };
 
+// Member m's view of the arguments: every array pointer moved to the member's part.
+template <typename R>
+__device__ __forceinline__ PlaneArgs plane_view(PlaneArgs a, int m) {
+  if (m == 0) return a;
+  const MemberStrides s = member_strides(a.C, a.nyu, a.nyv);
+  const int Kp = row_pitch<R>(a.Kx);
+  a.u1 += m * s.ru; a.u3 += m * s.ru; a.u2 += m * s.rv;
+  a.I = reinterpret_cast<cx<R> *>(a.I) + m * member_istride(a.bjcap, a.Mz, Kp);
+  a.P = reinterpret_cast<cx<R> *>(a.P) + m * member_pstride(a.bjcap, a.Mz, Kp);
+  a.Nxz1 = reinterpret_cast<cx<R> *>(a.Nxz1) + m * s.rp;
+  a.Nxz2 = reinterpret_cast<cx<R> *>(a.Nxz2) + m * s.rp;
+  a.Nxz3 = reinterpret_cast<cx<R> *>(a.Nxz3) + m * s.rp;
+  a.uvf = reinterpret_cast<cx<R> *>(a.uvf) + m * s.rv;
+  a.wvf = reinterpret_cast<cx<R> *>(a.wvf) + m * s.rv;
+  a.vvc = reinterpret_cast<cx<R> *>(a.vvc) + m * s.ru;
+  a.cfl += (size_t)m * 2 * CFL_SLOTS;
+  return a;
+}
+
+__device__ __forceinline__ NArr narr_at(NArr x, size_t off) {
+  x.p = x.f32 ? (void *)(static_cast<float2 *>(x.p) + off) : (void *)(static_cast<double2 *>(x.p) + off);
+  return x;
+}
+
+// Member m's view of the column arguments; its Re and dtv come from its device state.
+__device__ __forceinline__ ColArgs col_view(ColArgs a, int m) {
+  const MemberStrides s = member_strides(a.C, a.nyu, a.nyv);
+  if (m > 0) {
+    a.u1 += m * s.ru; a.u3 += m * s.ru; a.du1 += m * s.ru; a.du3 += m * s.ru; a.w3u += m * s.ru;
+    a.u2 += m * s.rv; a.du2 += m * s.rv; a.w3v += m * s.rv;
+    a.p += m * s.rp; a.psi += m * s.rp; a.div += m * s.rp; a.cP += m * s.rp;
+    a.Nu1 = narr_at(a.Nu1, m * s.rp); a.Nu2 = narr_at(a.Nu2, m * s.rp); a.Nu3 = narr_at(a.Nu3, m * s.rp);
+    a.Nxz1 = narr_at(a.Nxz1, m * s.rp); a.Nxz2 = narr_at(a.Nxz2, m * s.rp); a.Nxz3 = narr_at(a.Nxz3, m * s.rp);
+    a.uvf = narr_at(a.uvf, m * s.rv); a.wvf = narr_at(a.wvf, m * s.rv); a.vvc = narr_at(a.vvc, m * s.ru);
+    a.mscr += (size_t)m * 2 * (a.nyu + 2);
+    a.st += m;
+  }
+  a.Re = a.st->Re;
+  a.dtv = a.st->dtv;
+  return a;
+}
+
 bool plane_size_supported(int n);
 int xpass_rows_per_block(int Mx);  // rows of one x-pass block (largest variant); must divide Mz
 int plane_variants(int n);          // tile variants instantiated for transform length n
 // which: 0 z-inverse (nw = nbj + 1 field rows), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes (nw = nbj)
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);
-void launch_cfl_reduce(const double *slots, double *out, hipStream_t st);
+void launch_cfl_reduce(const double *slots, DnsState *out, int nm, hipStream_t st);
+void launch_cfl_reset(DnsState *st, double *slots, int nm, hipStream_t s);
 void launch_widen(const float2 *in, double2 *out, size_t n, hipStream_t st);
 void launch_narrow(const double2 *in, float2 *out, size_t n, hipStream_t st);
-void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st);
+void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, size_t ms, int nm, double *cfl, hipStream_t st);
 void launch_zero(double *p, int n, hipStream_t st);
 int launch_plane_roundtrip(int Mx, int Mz, const double *in, double *phys, double *back, int n,
                            const double2 *twx, const double2 *twh, const double2 *twz, hipStream_t st);

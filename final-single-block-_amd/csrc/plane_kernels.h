@@ -30,12 +30,14 @@ template <class PL, int N> constexpr int PL_M = N / plan_info<PL>::first;
 // -> u-plane r, interpolated with the row below -> v-plane r-1); u2 rows jc0-1..jc1 (plain -> v-plane r,
 // interpolated with the row below -> u-plane r).
 template <typename R, int MZ, int T, int NTH, class PL, int ZCH, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
+__global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
   using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
   __shared__ cx<R> s[L::SIZE];
   constexpr int KS = last_slots<L, NTH, PL>();
+  const int nch = (a_.nbj + ZCH - 1) / ZCH;  // grid z = chunks x ensemble members
+  const PlaneArgs a = plane_view<R>(a_, blockIdx.z / nch);
   const int f = blockIdx.y;
-  const int jc0 = a.j0 + blockIdx.z * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
+  const int jc0 = a.j0 + (blockIdx.z % nch) * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
   const int i0 = blockIdx.x * T;
   const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);
   const size_t fsz = (size_t)MZ * Kp;
@@ -104,7 +106,8 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
 // (u-planes: uu, uw, vv, ww in slots 0..3; v-planes: uv, wv in slots 0..1).  VPL selects the plane type;
 // the launch covers planes of one type only.
 template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
+__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
+  const PlaneArgs a = plane_view<R>(a_, blockIdx.y / a_.nbj);  // grid y = planes x ensemble members
   constexpr int H = MX / 2;
   constexpr int NS = RB * 4;
   constexpr int NP = VPL ? 2 : 4;
@@ -112,7 +115,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
   using LF = Lay<H, NS, RB * NP, NP, 4, PS, true>;  // forward: slots r*4 + p, p < NP
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
-  const int jj = blockIdx.y, j = a.j0 + jj;
+  const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
   const int z0 = blockIdx.x * RB;
   constexpr int Kx = MX / 3 + 1;   // retained x-modes Nx/2 + 1, Nx = 2 MX/3 (the launcher checks a.Kx)
   const int MZ = a.Mz;
@@ -196,12 +199,13 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
 // (ops_in_planes2, FOU3D.f90:1590-1640).  The kz = -Nz/2 row is inside der_x/der_z's zero band; the raw
 // fluxes keep it.
 template <typename R, int MZ, int T, int NTH, bool VPL, class PL, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
+__global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a_) {
+  const PlaneArgs a = plane_view<R>(a_, blockIdx.y / a_.nbj);  // grid y = planes x ensemble members
   using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
   __shared__ cx<R> s[L::SIZE];
   constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
   constexpr int NP = VPL ? 2 : 4;
-  const int jj = blockIdx.y, j = a.j0 + jj;
+  const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
   const int i0 = blockIdx.x * T;
   constexpr int Nz = 2 * MZ / 3;    // retained z-modes (the launcher checks a.Nz)
   const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);

@@ -6,8 +6,11 @@ namespace dnsk {
 
 // ghost plane 0 of u1/u3 is still spectral when dtc_calc reads it (FOU3D.f90:622-626 against the
 // allocation at start.f90:549-551): its |Re|, |Im| join the CFL maxima.
-__global__ void cfl_ghost_kernel(const double2 *u1, const double2 *u3, int C, double *cfl) {
+__global__ void cfl_ghost_kernel(const double2 *u1, const double2 *u3, int C, size_t ms, double *cfl) {
   __shared__ double red[4];
+  u1 += blockIdx.y * ms;  // ensemble member blockIdx.y (member stride ms)
+  u3 += blockIdx.y * ms;
+  cfl += (size_t)blockIdx.y * 2 * CFL_SLOTS;
   double m1 = 0.0, m3 = 0.0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     const double2 a = u1[c], b = u3[c];
@@ -24,8 +27,10 @@ __global__ void cfl_ghost_kernel(const double2 *u1, const double2 *u3, int C, do
 }
 
 // max over the slots -> out[0] = max|u1|, out[1] = max|u3| (one block of CFL_SLOTS threads)
-__global__ void cfl_reduce_kernel(const double *slots, double *out) {
+__global__ void cfl_reduce_kernel(const double *slots, DnsState *st) {
   const int l = threadIdx.x;
+  slots += (size_t)blockIdx.x * 2 * CFL_SLOTS;  // ensemble member blockIdx.x
+  double *out = &st[blockIdx.x].cfl[0];
   double m1 = slots[l], m3 = slots[CFL_SLOTS + l];
   static_assert(CFL_SLOTS == 64, "one wavefront");
   for (int off = 32; off > 0; off >>= 1) {
@@ -188,7 +193,7 @@ static void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
     zinv_v<R, MZ, 0>(a, nw, st);
   } else {
     (void)nw;
-    dim3 g((a.Kx + Z::T - 1) / Z::T, 3, (a.nbj + Z::ZCH - 1) / Z::ZCH);
+    dim3 g((a.Kx + Z::T - 1) / Z::T, 3, ((a.nbj + Z::ZCH - 1) / Z::ZCH) * a.nm);
     zinv_kernel<R, MZ, Z::T, Z::NTH, typename ZPlanT<MZ>::type, Z::ZCH, 1, 0><<<g, Z::NTH, 0, st>>>(a);
   }
 }
@@ -198,7 +203,7 @@ static void zfwd_v(const PlaneArgs &a, int nw, hipStream_t st) {
   if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
     zfwd_v<R, MZ, 0, VPL>(a, nw, st);
   } else {
-    dim3 g((a.Kx + Z::T - 1) / Z::T, nw);
+    dim3 g((a.Kx + Z::T - 1) / Z::T, nw * a.nm);
     zfwd_kernel<R, MZ, Z::T, Z::NTH, VPL, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
   }
 }
@@ -210,7 +215,7 @@ static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
     xpass_v<R, MX, 0, VPL>(a, nw, st);
   } else {
     if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // default tile: 4 rows divide Mz
-    dim3 g(a.Mz / X::RB, nw);
+    dim3 g(a.Mz / X::RB, nw * a.nm);
     using PL = typename XPlanT<H>::type;
     xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);
   }
@@ -266,14 +271,29 @@ int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int va
   return fp32 ? launch_t<float>(a, nw, which, variant, st) : launch_t<double>(a, nw, which, variant, st);
 }
 
-void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, double *cfl, hipStream_t st) {
+void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, size_t ms, int nm, double *cfl, hipStream_t st) {
   int nb = (C + 255) / 256;
   if (nb > 1024) nb = 1024;
-  cfl_ghost_kernel<<<nb, 256, 0, st>>>(u1, u3, C, cfl);
+  cfl_ghost_kernel<<<dim3(nb, nm), 256, 0, st>>>(u1, u3, C, ms, cfl);
 }
 
-void launch_cfl_reduce(const double *slots, double *out, hipStream_t st) {
-  cfl_reduce_kernel<<<1, CFL_SLOTS, 0, st>>>(slots, out);
+void launch_cfl_reduce(const double *slots, DnsState *out, int nm, hipStream_t st) {
+  cfl_reduce_kernel<<<nm, CFL_SLOTS, 0, st>>>(slots, out);
+}
+
+// start of kRK = 1: CFL maxima, max|div| and the slots of every member to zero
+__global__ void cfl_reset_kernel(DnsState *st, double *slots, int nm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) {
+    st[i].cfl[0] = 0.0;
+    st[i].cfl[1] = 0.0;
+    st[i].divmax = 0.0;
+  }
+  if (i < nm * 2 * CFL_SLOTS) slots[i] = 0.0;
+}
+void launch_cfl_reset(DnsState *st, double *slots, int nm, hipStream_t s) {
+  const int n = nm * 2 * CFL_SLOTS;
+  cfl_reset_kernel<<<(n + 255) / 256, 256, 0, s>>>(st, slots, nm);
 }
 
 void launch_zero(double *p, int n, hipStream_t st) { zero_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n); }

@@ -29,8 +29,8 @@ class Params(ctypes.Structure):
                 ("flag_ctpress", ctypes.c_int), ("precision", ctypes.c_int),
                 ("Re", ctypes.c_double), ("alp", ctypes.c_double), ("bet", ctypes.c_double),
                 ("CFL", ctypes.c_double), ("mpgx", ctypes.c_double), ("dyq", ctypes.c_double),
-                ("fixed_dt", ctypes.c_double), ("plane_batch", ctypes.c_int),
-                ("reserved", ctypes.c_int * 7)]
+                ("fixed_dt", ctypes.c_double), ("plane_batch", ctypes.c_int), ("members", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -43,7 +43,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_vcorr", "dns_meanflow_ctP", "dns_rk3_substep", "dns_rk3_step", "dns_rk3_step_graph",
            "dns_plane_fft_roundtrip", "dns_modes_to_planes", "dns_planes_to_modes", "dns_sync",
            "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
-           "dns_autotune"]
+           "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
+           "dns_get_member_scalars"]
 
 
 def lib():
@@ -61,6 +62,11 @@ def lib():
     L.dns_setup_arrays.argtypes = [vp] + [_dp] * 7
     L.dns_ygrid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int] + [_dp] * 4
     L.dns_upload_fields.argtypes = [vp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.dns_upload_member.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_double]
+    L.dns_download_member.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.dns_set_member_params.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.dns_get_member_scalars.argtypes = [vp, ctypes.c_int, _dp]
     for n in ("dns_download_fields",):
         getattr(L, n).argtypes = [vp, _dp, _dp, _dp, _dp]
     for n in ("dns_download_nonlinear", "dns_upload_nonlinear", "dns_download_rhs", "dns_download_flux"):
@@ -105,11 +111,12 @@ class DNS:
     """One channel on the GPU (one dns_handle)."""
 
     def __init__(self, Mx, Mz, ny, Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
-                 flag_ctpress=1, precision=PREC_FP64, fixed_dt=0.0, plane_batch=0):
+                 flag_ctpress=1, precision=PREC_FP64, fixed_dt=0.0, plane_batch=0, members=1):
         L = lib()
         self.prm = Params(Mx=Mx, Mz=Mz, ny=ny, ppp=ppp, flag_ctpress=flag_ctpress, precision=precision,
                           Re=Re, alp=alp, bet=bet, CFL=CFL, mpgx=mpgx, dyq=dyq, fixed_dt=fixed_dt,
-                          plane_batch=plane_batch)
+                          plane_batch=plane_batch, members=members)
+        self.members = max(1, members)
         h = ctypes.c_void_p()
         _chk(L.dns_create(ctypes.byref(self.prm), ctypes.byref(h)))
         self.h = h
@@ -143,20 +150,31 @@ class DNS:
         return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv, dyu2i=d2u, dyv2i=d2v, dtv=w[0], gw=w[1:3], gbc=w[3],
                     gi=w[4:6])
 
-    def upload(self, u1, u2, u3, p, t=0.0, it=0, QxT=0.0):
+    def _fields(self, u1, u2, u3, p):
         arrs = []
         for name, a in (("u1", u1), ("u2", u2), ("u3", u3), ("p", p)):
             a = np.ascontiguousarray(a, dtype=np.complex128)
             if a.shape != self.shape(name):
                 raise DnsError("%s has shape %s, expected %s" % (name, a.shape, self.shape(name)))
             arrs.append(a)
-        _chk(lib().dns_upload_fields(self.h, *[_p(a.view(np.float64)) for a in arrs], float(t), int(it),
-                                     float(QxT)))
+        return arrs
 
-    def download(self):
+    def upload(self, u1, u2, u3, p, t=0.0, it=0, QxT=0.0, member=None):
+        """Fields into every ensemble member (member=None) or into one member."""
+        arrs = self._fields(u1, u2, u3, p)
+        ptrs = [_p(a.view(np.float64)) for a in arrs]
+        if member is None:
+            _chk(lib().dns_upload_fields(self.h, *ptrs, float(t), int(it), float(QxT)))
+        else:
+            _chk(lib().dns_upload_member(self.h, int(member), *ptrs, float(t), int(it), float(QxT)))
+
+    def download(self, member=0):
         out = [np.zeros(self.shape(n), np.complex128) for n in ("u1", "u2", "u3", "p")]
-        _chk(lib().dns_download_fields(self.h, *[_p(a.view(np.float64)) for a in out]))
+        _chk(lib().dns_download_member(self.h, int(member), *[_p(a.view(np.float64)) for a in out]))
         return out
+
+    def set_member_params(self, member, Re, mpgx):
+        _chk(lib().dns_set_member_params(self.h, int(member), float(Re), float(mpgx)))
 
     def download_nonlinear(self):
         out = [np.zeros(self.shape(n), np.complex128) for n in ("Nu1", "Nu2", "Nu3")]
@@ -182,9 +200,9 @@ class DNS:
         _chk(lib().dns_download_pressure_correction(self.h, *[_p(a.view(np.float64)) for a in out]))
         return out
 
-    def scalars(self):
+    def scalars(self, member=0):
         s = np.zeros(10)
-        _chk(lib().dns_get_scalars(self.h, _p(s)))
+        _chk(lib().dns_get_member_scalars(self.h, int(member), _p(s)))
         return dict(zip(["t", "dt", "dti", "mpgx", "Qx", "dgx", "dtc", "QxT", "iter", "divmax"], s))
 
     def set_dt(self, dt):

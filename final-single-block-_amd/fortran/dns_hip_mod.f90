@@ -14,7 +14,8 @@ module dns_hip
     integer(c_int) :: Mx, Mz, ny, ppp, flag_ctpress, precision
     real(c_double) :: Re, alp, bet, CFL, mpgx, dyq, fixed_dt
     integer(c_int) :: plane_batch
-    integer(c_int) :: reserved(7)
+    integer(c_int) :: members
+    integer(c_int) :: reserved(6)
   end type
 
   interface
@@ -175,6 +176,31 @@ module dns_hip
       type(c_ptr), value :: h
       real(c_double) :: t(7)
       integer(c_int) :: n
+    end function
+    integer(c_int) function dns_upload_member(h, m, u1, u2, u3, p, t, iter, QxT) bind(C, name='dns_upload_member')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: m, iter
+      real(c_double) :: u1(*), u2(*), u3(*), p(*)
+      real(c_double), value :: t, QxT
+    end function
+    integer(c_int) function dns_download_member(h, m, u1, u2, u3, p) bind(C, name='dns_download_member')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: m
+      real(c_double) :: u1(*), u2(*), u3(*), p(*)
+    end function
+    integer(c_int) function dns_set_member_params(h, m, Re, mpgx) bind(C, name='dns_set_member_params')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: m
+      real(c_double), value :: Re, mpgx
+    end function
+    integer(c_int) function dns_get_member_scalars(h, m, s) bind(C, name='dns_get_member_scalars')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: m
+      real(c_double) :: s(10)
     end function
     integer(c_int) function dns_get_plane_variants(h, var, count) bind(C, name='dns_get_plane_variants')
       import :: c_int, c_ptr

@@ -128,7 +128,7 @@ program littleharsh_hip
   prm%Mx = Ngal_x; prm%Mz = Ngal_z; prm%ny = ny; prm%ppp = ppp
   prm%flag_ctpress = flag_ctpress; prm%precision = DNS_PREC_FP64
   prm%Re = Re; prm%alp = alp; prm%bet = bet; prm%CFL = CFL; prm%mpgx = mpgx; prm%dyq = dyq
-  prm%fixed_dt = 0d0; prm%plane_batch = 0; prm%reserved = 0
+  prm%fixed_dt = 0d0; prm%plane_batch = 0; prm%members = 1; prm%reserved = 0
   call dns_hip_check(dns_create(prm, hdns), 'start')
 
   allocate(u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol))

@@ -48,7 +48,9 @@ typedef struct dns_params {
   double fixed_dt;        /* > 0: use this dt instead of the reference's CFL rule (FOU3D.f90:290-303) */
   int plane_batch;        /* planes per plane-stage launch group (u- and v-planes counted, rounded up to
                              an even number); 0 = all planes */
-  int reserved[7];
+  int members;            /* ensemble members of these sizes (0 or 1: a single channel); each member has
+                             its own fields, Re, mpgx, dt and state, and every launch covers all members */
+  int reserved[6];
 } dns_params;
 
 const char *dns_last_error(void);
@@ -72,12 +74,22 @@ int dns_ygrid(int ny, double dyq, int ppp, double *yu, double *yv, double *dthdy
 
 /* getini (start.f90:1221-1316) after the fields are in memory: uploads u1,u2,u3,p (reference column
  * layout, host memory), sets t and iter, Qx, and QxT (QxT <= 0: QxT = flow rate of u1, as getini).
- * The previous-substep nonlinear terms are cleared (they are multiplied by cRK(1) = 0 at kRK = 1). */
+ * The previous-substep nonlinear terms are cleared (they are multiplied by cRK(1) = 0 at kRK = 1).
+ * dns_upload_fields loads the same fields into every ensemble member; dns_upload_member into member m. */
 int dns_upload_fields(dns_handle *h, const double *u1, const double *u2, const double *u3, const double *p,
                       double t, int iter, double QxT);
-/* Copies the current fields back into host arrays in the reference layout (record_out's input). */
+int dns_upload_member(dns_handle *h, int m, const double *u1, const double *u2, const double *u3,
+                      const double *p, double t, int iter, double QxT);
+/* Copies the current fields back into host arrays in the reference layout (record_out's input):
+ * member 0, or member m. */
 int dns_download_fields(dns_handle *h, double *u1, double *u2, double *u3, double *p);
-/* Nonlinear terms Nu1 (j=1..nyu), Nu2 (j=1..nyv), Nu3 (j=1..nyu) in the reference layout. */
+int dns_download_member(dns_handle *h, int m, double *u1, double *u2, double *u3, double *p);
+/* Ensemble member m's Reynolds number (and with it dtv, start.f90:476) and mean pressure gradient;
+ * the defaults are dns_params' Re and mpgx.  Takes effect at the next upload or step.  No counterpart
+ * in the reference (one channel per run); it serves the ensemble form of the step. */
+int dns_set_member_params(dns_handle *h, int m, double Re, double mpgx);
+/* The following transfers act on ensemble member 0.
+ * Nonlinear terms Nu1 (j=1..nyu), Nu2 (j=1..nyv), Nu3 (j=1..nyu) in the reference layout. */
 int dns_download_nonlinear(dns_handle *h, double *Nu1, double *Nu2, double *Nu3);
 int dns_upload_nonlinear(dns_handle *h, const double *Nu1, const double *Nu2, const double *Nu3);
 /* Right-hand sides du1 (j=1..nyu), du2 (j=1..nyv), du3 (j=1..nyu) left by dns_rhs0 / dns_nonlinear. */
@@ -87,8 +99,10 @@ int dns_download_flux(dns_handle *h, double *uv_f, double *vv_c, double *wv_f);
 /* psi and div (j = 1..nyu) of the last dns_solveP / dns_vcorr. */
 int dns_download_pressure_correction(dns_handle *h, double *psi, double *div);
 
-/* s[0..9] = t, dt, dti, mpgx, Qx, dgx, dtc, QxT, iter, max|div| (last dns_vcorr; 0 after fused steps) */
+/* s[0..9] = t, dt, dti, mpgx, Qx, dgx, dtc, QxT, iter, max|div| (last dns_vcorr or fused step) of
+ * member 0, or member m. */
 int dns_get_scalars(dns_handle *h, double *s);
+int dns_get_member_scalars(dns_handle *h, int m, double *s);
 /* Sets dt (and dti) for a call sequence that starts at kRK > 1 (the reference sets dt at kRK = 1). */
 int dns_set_dt(dns_handle *h, double dt);
 

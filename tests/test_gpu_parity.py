@@ -307,3 +307,95 @@ def test_autotune_choice_valid_and_results_unchanged():
     with pytest.raises(dnship.DnsError):
         h.set_plane_variants([0, 1, 0, 0, 0, 0, 0, 0])
     h.close()
+
+
+def test_restart_bit_exact():
+    """Checkpoint/restart at a step boundary (record_out/getini): download the fields, write and read
+    the reference's field files, upload into a fresh handle and continue; the result equals the
+    uninterrupted run bit for bit (at kRK = 1 the carried nonlinear terms have weight cRK = 0)."""
+    import os
+    import tempfile
+    import dns_io
+    Mx, Mz, ny = 96, 96, 49
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=9)
+    ref, _, sref = gpu_run(Mx, Mz, ny, st, 7)
+    g = dnship.DNS(Mx, Mz, ny)
+    g.upload(*st)
+    g.step(4)
+    g.sync()
+    mid = g.download()
+    s4 = g.scalars()
+    g.close()
+    gr = dnship.ygrid(ny)
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    N = np.zeros((4, 5), np.int32)
+    N[:, 3] = [Nx, Nz, ny - 2, ny - 1]
+    with tempfile.TemporaryDirectory() as d:
+        for name, f, j0, y, dth in (("u1", mid[0], 0, gr["yu"], gr["dthdyu"]), ("u2", mid[1], 0, gr["yv"], gr["dthdyv"]),
+                                    ("u3", mid[2], 0, gr["yu"], gr["dthdyu"]), ("p", mid[3], 1, gr["yu"], gr["dthdyu"])):
+            planes = dns_io.columns_to_planes(f, Nz)
+            js = np.arange(j0, j0 + planes.shape[0])
+            ff = dns_io.FieldFile(s4["t"], 3250.0, 1.0, 2.0, s4["mpgx"], 3, int(s4["iter"]), N, y, 1.0, dth, js,
+                                  planes, y[js])
+            dns_io.write_field(os.path.join(d, name + "_restart.dat"), ff)
+        u1, u2, u3, p, head = dns_io.load_state(d, "_restart.dat", ny)
+    h = dnship.DNS(Mx, Mz, ny, mpgx=head.mpgx)
+    h.upload(u1, u2, u3, p, t=head.t, it=head.iter)
+    h.step(3)
+    h.sync()
+    out = h.download()
+    s7 = h.scalars()
+    h.close()
+    for a, b in zip(out, ref):
+        assert np.array_equal(a, b)
+    assert s7["t"] == sref["t"] and s7["iter"] == sref["iter"] == 7
+
+
+@pytest.mark.parametrize("prec,graph,batch", [(dnship.PREC_FP64, False, 0), (dnship.PREC_FP64, True, 9),
+                                              (dnship.PREC_MIXED, False, 0)])
+def test_ensemble_members_equal_single_channels(prec, graph, batch):
+    """A batched ensemble (one launch per kernel for all members) gives every member exactly the bits of
+    the same channel run alone: own IC, own Re and mpgx, own CFL dt."""
+    Mx, Mz, ny, nsteps = 48, 48, 33, 3
+    params = [(3250.0, -3.0675e-3, 11), (2000.0, -4.0e-3, 12), (4500.0, -2.5e-3, 13)]
+    ics = [dns_ic.seeded_channel(Mx, Mz, ny, seed=sd) for _, _, sd in params]
+    e = dnship.DNS(Mx, Mz, ny, precision=prec, members=len(params), plane_batch=batch)
+    for m, ((Re, mp, _), st) in enumerate(zip(params, ics)):
+        e.set_member_params(m, Re, mp)
+        e.upload(*st, member=m)
+    print("case: ensemble of %d, prec %d, graph %s, batch %d" % (len(params), prec, graph, batch), flush=True)
+    if graph:
+        e.step_graph(nsteps)
+    else:
+        e.step(nsteps)
+    e.sync()
+    outs = [e.download(member=m) for m in range(len(params))]
+    scs = [e.scalars(member=m) for m in range(len(params))]
+    e.close()
+    for m, ((Re, mp, _), st) in enumerate(zip(params, ics)):
+        ref, _, sref = gpu_run(Mx, Mz, ny, st, nsteps, Re=Re, mpgx=mp, precision=prec, plane_batch=batch)
+        for x, y in zip(outs[m], ref):
+            assert np.array_equal(x, y), m
+        assert scs[m]["t"] == sref["t"] and scs[m]["dt"] == sref["dt"]
+    assert len({s["dt"] for s in scs}) == len(params)   # each member its own CFL step
+
+
+def test_ensemble_vs_oracle():
+    """Two members at different Re against the oracle (FP64 bar)."""
+    Mx, Mz, ny = 48, 48, 33
+    params = [(3250.0, -3.0675e-3, 21), (2500.0, -3.5e-3, 22)]
+    e = dnship.DNS(Mx, Mz, ny, members=2)
+    ics = []
+    for m, (Re, mp, sd) in enumerate(params):
+        st = dns_ic.seeded_channel(Mx, Mz, ny, seed=sd)
+        ics.append(st)
+        e.set_member_params(m, Re, mp)
+        e.upload(*st, member=m)
+    e.step(4)
+    e.sync()
+    for m, (Re, mp, sd) in enumerate(params):
+        out = e.download(member=m)
+        ref, _, sc, _ = oracle_run(Mx, Mz, ny, ics[m], 4, Re=Re, mpgx=mp)
+        for a, b in zip(out, ref):
+            assert rel_l2(a, b) <= TOL
+    e.close()
