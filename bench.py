@@ -49,6 +49,47 @@ def gpu_leg(st, prec, steps, warmup):
     return (t1 - t0) / steps, tim, sc, fields, tune
 
 
+# Ensemble workload: 24 x (32^2 x 153), 16 x (64^2 x 97), 8 x (128^2 x 153) modes; Re_tau per member
+# uniform in [150, 400] (seeded), bulk Re on the half height from Dean's correlation
+# Re_tau = 0.09 (2 Re)^0.88, mpgx = -(Re_tau/Re)^2; member IC seed = 100 + member index.
+ENSEMBLE = [(24, 48, 48, 153), (16, 96, 96, 97), (8, 192, 192, 153)]
+
+
+def ensemble_leg(steps, warmup, prec):
+    import numpy as np
+    import dnship
+    import dns_ic
+    import dns_roofline
+    rng = np.random.default_rng(7)
+    handles, by, idx = [], 0.0, 0
+    for nm, Mx, Mz, ny in ENSEMBLE:
+        g = dnship.DNS(Mx, Mz, ny, precision=prec, members=nm)
+        for m in range(nm):
+            ret = rng.uniform(150.0, 400.0)
+            Re = 0.5 * (ret / 0.09) ** (1.0 / 0.88)
+            g.set_member_params(m, Re, -(ret / Re) ** 2)
+            g.upload(*dns_ic.seeded_channel(Mx, Mz, ny, seed=100 + idx, amp=0.1), member=m)
+            idx += 1
+        g.autotune(iters=3)
+        handles.append(g)
+        by += nm * dns_roofline.kernel_bytes(Mx, Mz, ny, prec == dnship.PREC_MIXED)["step"]
+    for g in handles:
+        g.step(warmup)
+    for g in handles:
+        g.sync()
+    t0 = time.perf_counter()
+    for g in handles:          # one stream per handle: the three groups run concurrently
+        g.step(steps)
+    for g in handles:
+        g.sync()
+    sec = (time.perf_counter() - t0) / steps
+    for g in handles:
+        g.close()
+    return {"value": 1.0 / sec, "unit": "ensemble steps/s (all 48 members advance one RK3 step)",
+            "ms_per_step": sec * 1e3, "members": idx, "groups": [list(x) for x in ENSEMBLE],
+            "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS}
+
+
 def cpu_leg(st):
     """The reference's own hot path (oracle/_ref, compiled from its sources) on one host core, timed on a
     bounded sample: one RK3 substep of the same 384x384x257 workload; falls back to the oracle's C
@@ -85,6 +126,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
@@ -129,6 +171,10 @@ def main():
                         "kernels_ms": {k: timmx[k] for k in KERNELS},
                         "step_frac_hbm": bym["step"] / secmx / 1e9 / dns_roofline.HBM_PEAK_GBS}
         out["autotune"]["mixed"] = tunemx
+    if not args.no_ensemble:
+        out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}
+        if not args.no_mixed:
+            out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(st)
     print(json.dumps(out), flush=True)
