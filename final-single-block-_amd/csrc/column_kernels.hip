@@ -838,6 +838,89 @@ __global__ void __launch_bounds__(NTH) fused_final_kernel(ColArgs a_) {
   if (threadIdx.x == 0) atomic_max_nonneg(&a.st->divmax, dmax);
 }
 
+// The final sweep, thread per column, with the inputs of the next D rows in flight while a row is
+// computed (one wavefront per SIMD streams the fields, so its own loads must cover the latency).
+// Arithmetic identical to fused_final_kernel.
+template <int NTH, int D>
+__global__ void __launch_bounds__(NTH) fused_final_pf_kernel(ColArgs a_) {
+  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
+  __shared__ double red[NTH / 64];
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  double dmax = 0.0;
+  if (col < a.C) {
+    const int C = a.C, nyu = a.nyu, nyv = a.nyv;
+    const Mode m = col_mode(a, col);
+    const double dtrk = a.st->dt * (a.aRK + a.bRK);
+    const double kxd = dtrk * m.kx, kzd = dtrk * m.kz;
+    const bool qx = (col == 0) && (a.kRK == 3) && (a.flag_ctpress == 1);
+    double Q = 0.0;
+    double2 psm = d2(0.0, 0.0);
+    double2 u1prev = d2(0.0, 0.0), u3prev = d2(0.0, 0.0), v_m1 = d2(0.0, 0.0), v_m2 = d2(0.0, 0.0);
+    struct Row { double2 x, p, u1, u3, u2; double c; };
+    auto load = [&](int j) {
+      Row r;
+      const size_t o = (size_t)(j - 1) * C + col;
+      r.x = a.psi[o];
+      r.c = a.cP[o];
+      r.p = a.p[o];
+      r.u1 = a.u1[(size_t)j * C + col];
+      r.u3 = a.u3[(size_t)j * C + col];
+      r.u2 = j >= 2 ? a.u2[(size_t)(j - 1) * C + col] : d2(0.0, 0.0);
+      return r;
+    };
+    Row q[D + 1];
+#pragma unroll
+    for (int k = 0; k < D; k++) if (1 + k <= nyu) q[k] = load(1 + k);
+    for (int j0 = 1; j0 <= nyu; j0 += D + 1) {
+#pragma unroll
+      for (int k = 0; k <= D; k++) {
+        const int j = j0 + k;
+        if (j > nyu) break;
+        // issue the loads of row j + D into the slot freed by row j - 1
+        if (j + D <= nyu) q[(k + D) % (D + 1)] = load(j + D);
+        const Row &r = q[k];
+        const size_t o = (size_t)(j - 1) * C + col;
+        const double2 ps = (j == 1) ? r.x : csub(r.x, rmul(r.c, psm));
+        a.p[o] = cadd(r.p, ps);
+        const double2 u1n = csub(r.u1, imul(kxd, ps));
+        const double2 u3n = csub(r.u3, imul(kzd, ps));
+        a.u1[(size_t)j * C + col] = u1n;
+        a.u3[(size_t)j * C + col] = u3n;
+        if (j == 1) {
+          a.u1[col] = rmul(a.gbc, u1n);
+          a.u3[col] = rmul(a.gbc, u3n);
+          a.u2[col] = d2(0.0, 0.0);
+          v_m2 = d2(0.0, 0.0);
+        } else {  // v row j-1 in 1..nyv
+          const double2 vn = csub(r.u2, rmul(dtrk * 1.0 * a.dthdyv[j - 1], csub(ps, psm)));
+          a.u2[(size_t)(j - 1) * C + col] = vn;
+          v_m1 = vn;
+          const double2 dv = cadd(cadd(imul(m.kx, u1prev), imul(m.kz, u3prev)),
+                                  rmul(1.0 * a.dthdyu[j - 1], csub(v_m1, v_m2)));
+          dmax = fmax(dmax, hypot(dv.x, dv.y));
+          v_m2 = v_m1;
+        }
+        if (qx) Q = Q + (a.yv[j] - a.yv[j - 1]) * u1n.x;
+        u1prev = u1n;
+        u3prev = u3n;
+        psm = ps;
+      }
+    }
+    a.u2[(size_t)(nyv + 1) * C + col] = d2(0.0, 0.0);
+    a.u1[(size_t)(nyu + 1) * C + col] = rmul(a.gbc, u1prev);
+    a.u3[(size_t)(nyu + 1) * C + col] = rmul(a.gbc, u3prev);
+    {
+      const double2 dv = cadd(cadd(imul(m.kx, u1prev), imul(m.kz, u3prev)),
+                              rmul(1.0 * a.dthdyu[nyu], csub(d2(0.0, 0.0), v_m2)));
+      dmax = fmax(dmax, hypot(dv.x, dv.y));
+    }
+    if (qx) a.st->Qx = Q;
+    if (col == 0 && a.kRK == 3) a.st->iter = a.st->iter + 1.0;
+  }
+  dmax = block_max<NTH>(dmax, red);
+  if (threadIdx.x == 0) atomic_max_nonneg(&a.st->divmax, dmax);
+}
+
 // ================================================================================================
 // layout: reference column layout [col][row] <-> device layout [row][col], 32x32 LDS tiles
 __global__ void transpose_kernel(const double2 *in, double2 *out, int rows_in, int cols_in) {
@@ -906,8 +989,8 @@ void launch_fused_down(const ColArgs &a, int variant, hipStream_t st) {
   else fused_down_kernel<<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
 }
 void launch_fused_final(const ColArgs &a, int variant, hipStream_t st) {
-  if (variant == 1) fused_final4_kernel<4><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
-  else if (variant == 2) fused_final4_kernel<8><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
+  if (variant == 1) fused_final_pf_kernel<256, 2><<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
+  else if (variant == 2) fused_final4_kernel<4><<<dim3(nblk(a.C, 64), a.nm), 256, 0, st>>>(a);
   else fused_final_kernel<256><<<dim3(nblk(a.C, 256), a.nm), 256, 0, st>>>(a);
 }
 
