@@ -43,6 +43,7 @@ struct dns_handle {
   int bj;  // plane index j per plane-stage group (u- and v-planes of each j)
   // variant per kernel: z-inverse, x-pass u/v, z-forward u/v, column sweeps up/down/final
   int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool colpass = false;  // column stage: one-pass kernel (true) or the three sweeps
   size_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -221,6 +222,8 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
     return fail(DNS_ERR_ARG, "dns_create: flag_ctpress=%d must be 0 or 1", prm->flag_ctpress);
   if (prm->precision != DNS_PREC_FP64 && prm->precision != DNS_PREC_MIXED)
     return fail(DNS_ERR_ARG, "dns_create: unknown precision %d", prm->precision);
+  if (prm->column_path != 0 && prm->column_path != 1)
+    return fail(DNS_ERR_ARG, "dns_create: column_path=%d must be 0 or 1", prm->column_path);
 
   dns_handle *h = new dns_handle();
   h->prm = *prm;
@@ -230,6 +233,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   h->nyv = ny - 2; h->nyu = h->nyv + 1;
   h->fp32 = prm->precision == DNS_PREC_MIXED;
   h->nm = prm->members > 1 ? prm->members : 1;
+  h->colpass = prm->column_path == 0 && prm->flag_ctpress == 1 && dnsk::column_pass_supported(h->nyu);
   h->uploaded.assign(h->nm, 0);
   h->Re_m.assign(h->nm, prm->Re);
   h->mpgx_m.assign(h->nm, prm->mpgx);
@@ -696,6 +700,14 @@ static int fused_substep(dns_handle *h, int kRK) {
   int rc = plane_stage(h, kRK);
   if (rc) return rc;
   ColArgs a = col_args(h, kRK);
+  if (h->colpass) {
+    if (dnsk::launch_column_pass(a, h->stream)) return fail(DNS_ERR_ARG, "column pass: %d rows not instantiated", h->nyu);
+    tmark(h, 4);
+    tmark(h, 5);
+    tmark(h, 6);
+    HIPCHK(hipGetLastError());
+    return DNS_OK;
+  }
   dnsk::launch_fused_up(a, h->var[5], h->stream);
   tmark(h, 4);
   dnsk::launch_fused_down(a, h->var[6], h->stream);
@@ -831,7 +843,7 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
     double tbest = 1e300;
     for (int v = 0; v < 3; v++) {
       if (ms) ms[3 * c + v] = -1.0;
-      if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0)) continue;
+      if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0) || (c >= 5 && h->colpass)) continue;
       auto launch = [&]() -> int {
         if (c < 5) return dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
         if (c == 5) dnsk::launch_fused_up(ca, v, h->stream);
@@ -851,6 +863,7 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
       if (ms) ms[3 * c + v] = tv;
       if (tv < tbest) { tbest = tv; best = v; }
     }
+    if (c >= 5 && h->colpass) best = h->var[c];  // sweeps not in use: keep the current choice
     h->var[c] = best;
     if (chosen) chosen[c] = best;
   }
@@ -870,6 +883,21 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   drop_graph(h);
   if (rc == DNS_OK) HIPCHK(hipGetLastError());
   return rc;
+}
+
+extern "C" int dns_set_column_path(dns_handle *h, int path) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_set_column_path: null handle");
+  if (path != 0 && path != 1) return fail(DNS_ERR_ARG, "dns_set_column_path: path=%d must be 0 or 1", path);
+  h->prm.column_path = path;
+  h->colpass = path == 0 && h->prm.flag_ctpress == 1 && dnsk::column_pass_supported(h->nyu);
+  drop_graph(h);
+  return DNS_OK;
+}
+
+extern "C" int dns_get_column_path(dns_handle *h, int *path) {
+  if (!h || !path) return fail(DNS_ERR_ARG, "dns_get_column_path: null argument");
+  *path = h->colpass ? 0 : 1;
+  return DNS_OK;
 }
 
 extern "C" int dns_sync(dns_handle *h) {

@@ -171,6 +171,9 @@ void launch_qx(const ColArgs &a, hipStream_t st);
 void launch_fused_up(const ColArgs &a, int variant, hipStream_t st);  // 0 thread/column, 1-2 wave/solve
 void launch_fused_down(const ColArgs &a, int variant, hipStream_t st);   // 0 thread/column, 1-2 four waves
 void launch_fused_final(const ColArgs &a, int variant, hipStream_t st);  // 0 thread/column, 1 + prefetch, 2 four waves
+// one-pass column stage (column_pass.hip): flag_ctpress = 1, 8 <= nyu <= 512
+bool column_pass_supported(int nyu);
+int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered
 // layout
 void launch_ref_to_dev(const double2 *ref, double2 *dev, int C, int nrows, hipStream_t st);
 void launch_dev_to_ref(const double2 *dev, double2 *ref, int C, int nrows, hipStream_t st);

@@ -43,7 +43,15 @@ def kernel_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
     # final sweep: reads psi, cP, p (1..nyu), u1, u3 (1..nyu), u2 (1..nyv); writes p, u1, u3, u2 (all rows)
     b["col_final"] = ((nyu + nyu + 2 * nyu + nyv) * S + nyu * R
                       + (nyu + 2 * (nyu + 2) + (nyv + 2)) * S)
-    b["substep"] = sum(b[k] for k in ("zinv", "xpass", "zfwd", "col_up", "col_down", "col_final"))
+    # one-pass column stage: every field read once and written once.  Reads u1, u3 (0..nyu+1), u2
+    # (0..nyv+1), p, the old Nu, Nxz, uv_f, wv_f (0..nyv+1), vv_c (1..nyu+1); writes u1, u3, u2, p, Nu.
+    # Nu, Nxz and the fluxes are stored in the plane precision.
+    Sf = (8 if fp32 else 16) * C
+    vel = (2 * (nyu + 2) + (nyv + 2) + nyu) * S
+    b["col_pass"] = (2 * vel + 2 * (2 * nyu + nyv) * Sf + (2 * (nyv + 2) + (nyu + 1)) * Sf
+                     + (2 * nyu + nyv) * Sf)
+    b["substep_sweeps"] = sum(b[k] for k in ("zinv", "xpass", "zfwd", "col_up", "col_down", "col_final"))
+    b["substep"] = b["zinv"] + b["xpass"] + b["zfwd"] + b["col_pass"]
     b["step"] = 3 * b["substep"]
     return b
 
@@ -54,3 +62,22 @@ def roofline(kernel: str, Mx: int, Mz: int, ny: int, fp32: bool, avg_ms: float) 
     gbs = by / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
             "bytes_per_launch": by}
+
+
+def reference_design_step_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> float:
+    """Compulsory HBM bytes of one RK3 step of the reference's own decomposition (columns -> 12 transposes
+    -> plane FFTs -> column solves), per substep:
+      transposes (3nu + 3nv) 2S twice, z-inverse (3nu + 3nv)(S + I), fused x-pass nu 7I + nv 5I,
+      z-forward nu (4I + 3S) + nv (2I + 3S), column stage 21.5 Cb,
+    with S = Kx Nz 2s_f, I = Kx Mz 2s_f (s_f = 8 FP64, 4 mixed), Cb = Kx Nz ny 16, nu = ny - 1, nv = ny - 2.
+    The device layout here needs no transposes, so its own compulsory bytes (kernel_bytes) are lower;
+    this figure prices a step the way the reference's data movement would."""
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    Kx = Nx // 2 + 1
+    sf = 4 if fp32 else 8
+    S, I = Kx * Nz * 2 * sf, Kx * Mz * 2 * sf
+    Cb = Kx * Nz * ny * 16
+    nu, nv = ny - 1, ny - 2
+    sub = (2 * (3 * nu + 3 * nv) * 2 * S + (3 * nu + 3 * nv) * (S + I) + nu * 7 * I + nv * 5 * I
+           + nu * (4 * I + 3 * S) + nv * (2 * I + 3 * S) + 21.5 * Cb)
+    return 3.0 * sub

@@ -30,7 +30,7 @@ class Params(ctypes.Structure):
                 ("Re", ctypes.c_double), ("alp", ctypes.c_double), ("bet", ctypes.c_double),
                 ("CFL", ctypes.c_double), ("mpgx", ctypes.c_double), ("dyq", ctypes.c_double),
                 ("fixed_dt", ctypes.c_double), ("plane_batch", ctypes.c_int), ("members", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("column_path", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -44,7 +44,7 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_plane_fft_roundtrip", "dns_modes_to_planes", "dns_planes_to_modes", "dns_sync",
            "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
-           "dns_get_member_scalars"]
+           "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path"]
 
 
 def lib():
@@ -87,6 +87,8 @@ def lib():
     L.dns_get_plane_variants.argtypes = [vp, _ip, _ip]
     L.dns_set_plane_variants.argtypes = [vp, _ip]
     L.dns_autotune.argtypes = [vp, ctypes.c_int, _ip, _dp]
+    L.dns_set_column_path.argtypes = [vp, ctypes.c_int]
+    L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
 
@@ -111,11 +113,12 @@ class DNS:
     """One channel on the GPU (one dns_handle)."""
 
     def __init__(self, Mx, Mz, ny, Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
-                 flag_ctpress=1, precision=PREC_FP64, fixed_dt=0.0, plane_batch=0, members=1):
+                 flag_ctpress=1, precision=PREC_FP64, fixed_dt=0.0, plane_batch=0, members=1,
+                 column_path=0):
         L = lib()
         self.prm = Params(Mx=Mx, Mz=Mz, ny=ny, ppp=ppp, flag_ctpress=flag_ctpress, precision=precision,
                           Re=Re, alp=alp, bet=bet, CFL=CFL, mpgx=mpgx, dyq=dyq, fixed_dt=fixed_dt,
-                          plane_batch=plane_batch, members=members)
+                          plane_batch=plane_batch, members=members, column_path=column_path)
         self.members = max(1, members)
         h = ctypes.c_void_p()
         _chk(L.dns_create(ctypes.byref(self.prm), ctypes.byref(h)))
@@ -255,6 +258,16 @@ class DNS:
         ms = np.zeros(3 * self.NKERN)
         _chk(lib().dns_autotune(self.h, iters, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms)))
         return ch, ms.reshape(self.NKERN, 3)
+
+    def set_column_path(self, path):
+        """0: one-pass column kernel where it applies, 1: three sweeps."""
+        _chk(lib().dns_set_column_path(self.h, int(path)))
+
+    def column_path(self):
+        """Column path in use: 0 one pass, 1 three sweeps."""
+        v = ctypes.c_int()
+        _chk(lib().dns_get_column_path(self.h, ctypes.byref(v)))
+        return v.value
 
     # ---- device-pointer utilities (torch tensors provide the device memory)
     def plane_fft_roundtrip(self, d_in, d_phys, d_back, n):

@@ -15,7 +15,8 @@ module dns_hip
     real(c_double) :: Re, alp, bet, CFL, mpgx, dyq, fixed_dt
     integer(c_int) :: plane_batch
     integer(c_int) :: members
-    integer(c_int) :: reserved(6)
+    integer(c_int) :: column_path
+    integer(c_int) :: reserved(5)
   end type
 
   interface
@@ -161,6 +162,16 @@ module dns_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: h, d_planes, d_cols
       integer(c_int), value :: nrows
+    end function
+    integer(c_int) function dns_set_column_path(h, path) bind(C, name='dns_set_column_path')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: path
+    end function
+    integer(c_int) function dns_get_column_path(h, path) bind(C, name='dns_get_column_path')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: path
     end function
     integer(c_int) function dns_sync(h) bind(C, name='dns_sync')
       import :: c_int, c_ptr

@@ -50,7 +50,9 @@ typedef struct dns_params {
                              an even number); 0 = all planes */
   int members;            /* ensemble members of these sizes (0 or 1: a single channel); each member has
                              its own fields, Re, mpgx, dt and state, and every launch covers all members */
-  int reserved[6];
+  int column_path;        /* column stage: 0 one kernel per substep (held on chip; used when flag_ctpress = 1
+                             and 8 <= ny-1 <= 512, else the sweeps), 1 three sweeps */
+  int reserved[5];
 } dns_params;
 
 const char *dns_last_error(void);
@@ -135,12 +137,20 @@ int dns_modes_to_planes(dns_handle *h, const double *d_cols, int nrows, double *
 /* planes_to_modes_UVP (FOU3D.f90:1694-1824): the inverse gather. */
 int dns_planes_to_modes(dns_handle *h, const double *d_planes, int nrows, double *d_cols);
 
+/* Column stage of the fused path: path 0 = one kernel per substep (rows split into segments, tridiagonal
+ * systems by the partition method, everything read and written once), 1 = three sweeps (thread per
+ * column, reference elimination order).  Same equations as RHS0 + nonlinear (column part) + solveU +
+ * solveP + v_corr (littleharsh.f90:163-233); results agree to rounding.  set: requested path (path 0
+ * falls back to the sweeps where it does not apply); get: the path in use. */
+int dns_set_column_path(dns_handle *h, int path);
+int dns_get_column_path(dns_handle *h, int *path);
+
 /* Waits for all work on the handle's stream. */
 int dns_sync(dns_handle *h);
 /* Average device duration (ms) per substep of each kernel, from HIP events on the handle's stream,
  * over the dns_rk3_step calls made since dns_set_timing(h, 1): t[0] z-inverse, t[1] x-pass,
  * t[2] z-forward, t[3] column sweep up, t[4] column sweep down, t[5] final sweep (+ mean-flow
- * correction), t[6] whole substep.  With plane_batch below the plane count t[0] holds the whole plane
+ * correction), t[6] whole substep; with the one-pass column stage t[3] holds it and t[4], t[5] are 0.  With plane_batch below the plane count t[0] holds the whole plane
  * stage and t[1], t[2] are 0.  n = number of substeps averaged. */
 int dns_set_timing(dns_handle *h, int on);
 int dns_get_timing(dns_handle *h, double *t, int *n);

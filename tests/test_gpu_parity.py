@@ -38,12 +38,13 @@ def oracle_run(Mx, Mz, ny, st, nsteps, **kw):
     return o.get_state(), o.get_nu(), o.scalars(), hist
 
 
+@pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("ctp", [1, 0])
-def test_fused_steps_vs_reference_golden(golden, ctp):
+def test_fused_steps_vs_reference_golden(golden, ctp, path):
     g = golden("golden_steps_24x24x17.npz")
     st = [g[k + "_in"] for k in ("u1", "u2", "u3", "p")]
-    print("case: fused 24x24x17 10 steps ctp=%d" % ctp, flush=True)
-    out, nu, sc = gpu_run(24, 24, 17, st, 10, flag_ctpress=ctp)
+    print("case: fused 24x24x17 10 steps ctp=%d column path %d" % (ctp, path), flush=True)
+    out, nu, sc = gpu_run(24, 24, 17, st, 10, flag_ctpress=ctp, column_path=path)
     for k, a in zip(("u1", "u2", "u3", "p"), out):
         assert rel_l2(a, g["%s_ctp%d" % (k, ctp)]) <= TOL, k
     for k, a in zip(("Nu1", "Nu2", "Nu3"), nu):
@@ -58,11 +59,14 @@ def test_fused_steps_vs_reference_golden(golden, ctp):
     (192, 192, 153, 3, 3),    # 128 x 128 x 153
     (576, 576, 17, 2, 5),     # the 384 x 384 plane transforms (tuned radix plans), few planes
     (1152, 576, 9, 1, 6),     # 768 x 384 plane transforms
+    (24, 24, 257, 4, 7),      # the 257-point y grid of the 384 x 384 x 257 workload
+    (24, 24, 385, 2, 8),      # 385 points (one-pass column stage with 6-row segments)
 ])
-def test_fused_steps_vs_oracle(Mx, Mz, ny, nsteps, seed):
-    print("case: fused %dx%dx%d %d steps" % (Mx, Mz, ny, nsteps), flush=True)
+@pytest.mark.parametrize("path", [0, 1])
+def test_fused_steps_vs_oracle(Mx, Mz, ny, nsteps, seed, path):
+    print("case: fused %dx%dx%d %d steps column path %d" % (Mx, Mz, ny, nsteps, path), flush=True)
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=seed)
-    out, nu, sc = gpu_run(Mx, Mz, ny, st, nsteps)
+    out, nu, sc = gpu_run(Mx, Mz, ny, st, nsteps, column_path=path)
     ref, rnu, rsc, hist = oracle_run(Mx, Mz, ny, st, nsteps)
     for k, a, b in zip(("u1", "u2", "u3", "p"), out, ref):
         assert rel_l2(a, b) <= TOL, k
@@ -264,7 +268,7 @@ def test_plane_tile_variants_bit_identical(prec, Mx, Mz, ny):
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=4, broadband=True)
     ref = None
     for var in ([0] * 8, [1] * 8, [2] * 8, [1, 2, 0, 2, 1, 2, 1, 2]):
-        g = dnship.DNS(Mx, Mz, ny, precision=prec, plane_batch=11)
+        g = dnship.DNS(Mx, Mz, ny, precision=prec, plane_batch=11, column_path=1)
         cur, cnt = g.plane_variants()
         assert list(cur) == [0] * 8 and list(cnt) == [3] * 8
         print("case: variants %s prec %d" % (var, prec), flush=True)
@@ -284,8 +288,8 @@ def test_plane_tile_variants_bit_identical(prec, Mx, Mz, ny):
 def test_autotune_choice_valid_and_results_unchanged():
     Mx, Mz, ny = 192, 192, 33
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=5)
-    a, _, _ = gpu_run(Mx, Mz, ny, st, 2)
-    g = dnship.DNS(Mx, Mz, ny)
+    a, _, _ = gpu_run(Mx, Mz, ny, st, 2, column_path=1)
+    g = dnship.DNS(Mx, Mz, ny, column_path=1)   # the column sweeps' variants are timed on this path
     g.upload(*st)
     ch, ms = g.autotune(iters=2)
     assert np.all((ch >= 0) & (ch < 3))
@@ -307,6 +311,35 @@ def test_autotune_choice_valid_and_results_unchanged():
     with pytest.raises(dnship.DnsError):
         h.set_plane_variants([0, 1, 0, 0, 0, 0, 0, 0])
     h.close()
+
+
+@pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
+@pytest.mark.parametrize("Mx,Mz,ny,ctp", [(48, 48, 257, 1), (96, 96, 129, 1), (24, 24, 33, 0)])
+def test_column_paths_agree(prec, Mx, Mz, ny, ctp):
+    """One-pass column stage against the three sweeps on one handle (same plane stage); the one-pass
+    kernel is used where it applies (constant pressure gradient, 8 <= ny-1 <= 512), else both are the
+    sweeps and agree bit for bit."""
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=11, broadband=True)
+    print("case: column paths %dx%dx%d ctp=%d prec %d" % (Mx, Mz, ny, ctp, prec), flush=True)
+    outs = []
+    for path in (0, 1):
+        g = dnship.DNS(Mx, Mz, ny, precision=prec, flag_ctpress=ctp, column_path=path)
+        used = g.column_path()
+        assert used == (path if (ctp == 1 and 8 <= ny - 1 <= 512) else 1)
+        g.upload(*st)
+        g.step(3)
+        g.sync()
+        outs.append((g.download(), g.download_nonlinear(), g.scalars(), used))
+        g.close()
+    (a, na, sa, ua), (b, nb, sb, ub) = outs
+    # FP32 plane stage: last-bit differences of the FP64 fields flip FP32 roundings, so two mixed runs
+    # agree to the mixed form's own error (about 1e-7 after a few steps), not to FP64 rounding
+    tol = 0.0 if ua == ub else TOL if prec == dnship.PREC_FP64 else 1e-6
+    for x, y in zip(a + na, b + nb):
+        assert rel_l2(x, y) <= tol
+    assert abs(sa["t"] - sb["t"]) <= 1e-14 * sb["t"] and sa["iter"] == sb["iter"] == 3
+    assert abs(sa["Qx"] - sb["Qx"]) <= 1e-12 * abs(sb["Qx"])
+    assert sa["divmax"] < 1e-9 and sb["divmax"] < 1e-9
 
 
 def test_restart_bit_exact():
