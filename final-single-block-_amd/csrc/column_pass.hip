@@ -267,84 +267,81 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
   Part<NC, P, M, 1> pv;  // u2
   Segment<M, 2> &su = pu.sg;
   Segment<M, 1> &sv = pv.sg;
-  {  // u-grid rows: u1, u3
+  {
+    // Both grids' rows in one loop: every load is in bounds (clamped rows are masked in the
+    // arithmetic), so a row's loads of both grids are issued together.
     double2 u1m = a.u1[at(cu(j0 - 1))], u1c = a.u1[at(cu(j0))];
     double2 u3m = a.u3[at(cu(j0 - 1))], u3c = a.u3[at(cu(j0))];
     double2 uvm = uvf[at(cv(j0 - 1))], wvm = wvf[at(cv(j0 - 1))];
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-      const int j = j0 + r;
-      const bool ur = j <= nyu;
-      const int ju = cp(j);
-      const double2 u1p = a.u1[at(cu(j + 1))], u3p = a.u3[at(cu(j + 1))];
-      const double2 uvc = uvf[at(cv(j))], wvc = wvf[at(cv(j))];
-      const size_t oq = at(ju - 1);
-      const double2 pc = a.p[oq];
-      const double2 nu1 = Nu1[oq], nu3 = Nu3[oq], nx1 = Nxz1[oq], nx3 = Nxz3[oq];
-      const double g1 = D2U(1, ju), g2 = D2U(2, ju), g3 = D2U(3, ju), hu = a.dthdyu[ju];
-      const double dd2 = k2 + g2;
-      const double2 L1 = cadd(cadd(rmul(g1, u1m), rmul(dd2, u1c)), rmul(g3, u1p));
-      const double2 L3 = cadd(cadd(rmul(g1, u3m), rmul(dd2, u3c)), rmul(g3, u3p));
-      double2 r1 = cadd(cadd(rmul(C1, L1), rmul(C2, nu1)), imul(c3x, pc));
-      double2 r3 = cadd(cadd(rmul(C1, L3), rmul(C2, nu3)), imul(c3z, pc));
-      r1.x = r1.x + Cm;
-      const double2 N1 = cadd(nx1, rmul(hu, csub(uvc, uvm)));
-      const double2 N3 = cadd(nx3, rmul(hu, csub(wvc, wvm)));
-      r1 = cadd(r1, rmul(Cg, N1));
-      r3 = cadd(r3, rmul(Cg, N3));
-      if (live && ur) {
-        Nu1.set(oq, N1);
-        Nu3.set(oq, N3);
-      }
-      double au = -beta * g1, bu = 1.0 - beta * (g2 + m.k2x + m.k2z), cc = -beta * g3;
-      if (j == 1) {        // ghost row u_0 = -gw(1) u_1 folded into row 1
-        bu = bu - au * a.gw1;
-        au = 0.0;
-      }
-      if (j == nyu) {      // ghost row u_{nyu+1} = -gw(2) u_nyu folded into row nyu
-        bu = bu - cc * a.gw2;
-        cc = 0.0;
-      }
-      const double2 rr[2] = {ur ? d2(u1c.x + dt * r1.x, u1c.y + dt * r1.y) : z2,
-                             ur ? d2(u3c.x + dt * r3.x, u3c.y + dt * r3.y) : z2};
-      su.row(r, ur ? au : 0.0, ur ? bu : 1.0, ur ? cc : 0.0, rr);
-      u1m = u1c; u1c = u1p;
-      u3m = u3c; u3c = u3p;
-      uvm = uvc; wvm = wvc;
-      // one row's loads in flight at a time: hoisting all rows' loads runs out of registers
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-  {  // v-grid rows: u2
     double2 u2m = a.u2[at(cv(j0 - 1))], u2c = a.u2[at(cv(j0))];
     double2 vvc_c = vvc[at(cu(j0))];
     double2 pc = a.p[at(cp(j0) - 1)];
 #pragma unroll
     for (int r = 0; r < M; r++) {
       const int j = j0 + r;
-      const bool vr = j <= nyv;
-      const int jv = cq(j);
+      const bool ur = j <= nyu, vr = j <= nyv;
+      const int ju = cp(j), jv = cq(j);
+      // u-grid row j
+      const double2 u1p = a.u1[at(cu(j + 1))], u3p = a.u3[at(cu(j + 1))];
+      const double2 uvc = uvf[at(cv(j))], wvc = wvf[at(cv(j))];
+      const size_t oq = at(ju - 1);
+      const double2 nu1 = Nu1[oq], nu3 = Nu3[oq], nx1 = Nxz1[oq], nx3 = Nxz3[oq];
+      const double g1 = D2U(1, ju), g2 = D2U(2, ju), g3 = D2U(3, ju), hu = a.dthdyu[ju];
+      double2 N1, N2, N3;
+      {
+        const double dd2 = k2 + g2;
+        const double2 L1 = cadd(cadd(rmul(g1, u1m), rmul(dd2, u1c)), rmul(g3, u1p));
+        const double2 L3 = cadd(cadd(rmul(g1, u3m), rmul(dd2, u3c)), rmul(g3, u3p));
+        double2 r1 = cadd(cadd(rmul(C1, L1), rmul(C2, nu1)), imul(c3x, pc));
+        double2 r3 = cadd(cadd(rmul(C1, L3), rmul(C2, nu3)), imul(c3z, pc));
+        r1.x = r1.x + Cm;
+        N1 = cadd(nx1, rmul(hu, csub(uvc, uvm)));
+        N3 = cadd(nx3, rmul(hu, csub(wvc, wvm)));
+        r1 = cadd(r1, rmul(Cg, N1));
+        r3 = cadd(r3, rmul(Cg, N3));
+        double au = -beta * g1, bu = 1.0 - beta * (g2 + m.k2x + m.k2z), cc = -beta * g3;
+        // ghost rows folded in: u_0 = -gw(1) u_1 into row 1, u_{nyu+1} = -gw(2) u_nyu into row nyu
+        const double bf = bu - au * a.gw1;
+        bu = j == 1 ? bf : bu;
+        au = j == 1 ? 0.0 : au;
+        const double bl = bu - cc * a.gw2;
+        bu = j == nyu ? bl : bu;
+        cc = j == nyu ? 0.0 : cc;
+        const double2 rr[2] = {ur ? d2(u1c.x + dt * r1.x, u1c.y + dt * r1.y) : z2,
+                               ur ? d2(u3c.x + dt * r3.x, u3c.y + dt * r3.y) : z2};
+        su.row(r, ur ? au : 0.0, ur ? bu : 1.0, ur ? cc : 0.0, rr);
+      }
+      u1m = u1c; u1c = u1p;
+      u3m = u3c; u3c = u3p;
+      uvm = uvc; wvm = wvc;
+      // v-grid row j
       const double2 u2p = a.u2[at(cv(j + 1))];
       const double2 pn = a.p[at(cp(j + 1) - 1)];
       const double2 vvn = vvc[at(cu(j + 1))];
       const size_t ov = at(jv - 1);
       const double2 nu2 = Nu2[ov], nx2 = Nxz2[ov];
       const double e1 = D2V(1, jv), e2 = D2V(2, jv), e3 = D2V(3, jv), hv = a.dthdyv[jv];
-      const double2 L2 = cadd(cadd(rmul(e1, u2m), rmul(k2 + e2, u2c)), rmul(e3, u2p));
-      const double C3 = -((a.dRK * 1.0) * hv);
-      double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, nu2)), rmul(C3, csub(pn, pc)));
-      const double2 N2 = cadd(nx2, rmul(hv, csub(vvn, vvc_c)));
-      r2 = cadd(r2, rmul(Cg, N2));
+      {
+        const double2 L2 = cadd(cadd(rmul(e1, u2m), rmul(k2 + e2, u2c)), rmul(e3, u2p));
+        const double C3 = -((a.dRK * 1.0) * hv);
+        double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, nu2)), rmul(C3, csub(pn, pc)));
+        N2 = cadd(nx2, rmul(hv, csub(vvn, vvc_c)));
+        r2 = cadd(r2, rmul(Cg, N2));
+        const double av = j == 1 ? 0.0 : -beta * e1;    // wall v_0 = 0
+        const double bv = 1.0 - beta * (e2 + m.k2x + m.k2z);
+        const double cw = j == nyv ? 0.0 : -beta * e3;  // wall v_{nyv+1} = 0
+        const double2 rv = vr ? d2(u2c.x + dt * r2.x, u2c.y + dt * r2.y) : z2;
+        sv.row(r, vr ? av : 0.0, vr ? bv : 1.0, vr ? cw : 0.0, &rv);
+      }
+      // stores after both grids' loads of the row (a store behind a branch ends the batch of loads)
+      if (live && ur) {
+        Nu1.set(oq, N1);
+        Nu3.set(oq, N3);
+      }
       if (live && vr) Nu2.set(ov, N2);
-      const double av = j == 1 ? 0.0 : -beta * e1;    // wall v_0 = 0
-      const double bv = 1.0 - beta * (e2 + m.k2x + m.k2z);
-      const double cw = j == nyv ? 0.0 : -beta * e3;  // wall v_{nyv+1} = 0
-      const double2 rv = vr ? d2(u2c.x + dt * r2.x, u2c.y + dt * r2.y) : z2;
-      sv.row(r, vr ? av : 0.0, vr ? bv : 1.0, vr ? cw : 0.0, &rv);
       pc = pn;
       vvc_c = vvn;
       u2m = u2c; u2c = u2p;
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
   pu.first(sh.U, c, s);
@@ -508,7 +505,7 @@ int launch_column_pass(const ColArgs &a, hipStream_t st) {
     case 32: launch_pass<8, 8, 4>(a, st); break;
     case 64: launch_pass<8, 16, 4>(a, st); break;
     case 128: launch_pass<8, 32, 4>(a, st); break;
-    case 256: launch_pass<4, 64, 4>(a, st); break;
+    case 256: launch_pass<8, 64, 4>(a, st); break;
     case 384: launch_pass<4, 64, 6>(a, st); break;
     case 512: launch_pass<4, 64, 8>(a, st); break;
     default: return 1;
