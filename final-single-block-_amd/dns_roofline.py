@@ -20,7 +20,8 @@ def kernel_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
     C = Kx * Nz
     nyv = ny - 2
     nyu = nyv + 1
-    S = 16 * C                 # one spectral row
+    S = 16 * C                 # one spectral row (FP64 complex)
+    Sf = (8 if fp32 else 16) * C   # one row of Nu, Nxz or a flux: stored in the plane precision
     R = 8 * C                  # one row of real per-column scratch
     I = Kx * Mz * (8 if fp32 else 16)   # one plane-stage field
     npl = 2 * nyu              # u-planes 1..nyu, v-planes 1..nyv+1
@@ -30,11 +31,11 @@ def kernel_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
     # x-pass: 3 fields per plane in; 4 products per u-plane, 2 per v-plane out
     b["xpass"] = 3 * npl * I + (4 * nyu + 2 * nyu) * I
     # z-forward: products in; Nxz1, Nxz3, vv_c per u-plane; Nxz2 (j <= nyv), uv_f, wv_f per v-plane
-    b["zfwd"] = (4 * nyu + 2 * nyu) * I + (3 * nyu + 2 * nyu + nyv) * S
-    # column sweep up: reads u1,u3 (nyu+2), u2 (nyv+2), p, Nu1/3 old, Nxz1/3, vv_c (nyu each), Nu2 old, Nxz2
-    # (nyv each), uv_f, wv_f (nyu+1 each); writes u1,u3,u2 eliminated, Nu1/2/3 new, w3u, w3v
-    rd = (2 * (nyu + 2) + (nyv + 2) + 6 * nyu + 2 * nyv + 2 * (nyu + 1)) * S
-    wr = (2 * (nyu + 2) + (nyv + 2) + 2 * nyu + nyv) * S + ((nyu + 2) + (nyv + 2)) * R
+    b["zfwd"] = (4 * nyu + 2 * nyu) * I + (3 * nyu + 2 * nyu + nyv) * Sf
+    # column sweep up: reads u1,u3 (nyu+2), u2 (nyv+2), p (nyu); Nu1/3 old, Nxz1/3, vv_c (nyu each), Nu2
+    # old, Nxz2 (nyv each), uv_f, wv_f (nyu+1 each); writes u1,u3,u2 eliminated, Nu1/2/3 new, w3u, w3v
+    rd = (2 * (nyu + 2) + (nyv + 2) + nyu) * S + (5 * nyu + 2 * nyv + 2 * (nyu + 1)) * Sf
+    wr = (2 * (nyu + 2) + (nyv + 2)) * S + (2 * nyu + nyv) * Sf + ((nyu + 2) + (nyv + 2)) * R
     b["col_up"] = rd + wr
     # column sweep down: reads u1,u3,u2 (eliminated) and w3u, w3v; writes u1,u3 (0..nyu), u2 (0..nyv),
     # psi (1..nyu), cP
@@ -46,7 +47,6 @@ def kernel_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
     # one-pass column stage: every field read once and written once.  Reads u1, u3 (0..nyu+1), u2
     # (0..nyv+1), p, the old Nu, Nxz, uv_f, wv_f (0..nyv+1), vv_c (1..nyu+1); writes u1, u3, u2, p, Nu.
     # Nu, Nxz and the fluxes are stored in the plane precision.
-    Sf = (8 if fp32 else 16) * C
     vel = (2 * (nyu + 2) + (nyv + 2) + nyu) * S
     b["col_pass"] = (2 * vel + 2 * (2 * nyu + nyv) * Sf + (2 * (nyv + 2) + (nyu + 1)) * Sf
                      + (2 * nyu + nyv) * Sf)
