@@ -145,10 +145,13 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
 
   R m1 = 0, m3 = 0;
   const bool cfl = (a.kRK == 1) && !VPL && (j <= a.nyu - 1);
-  for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
-    const int r = idx / H, n = idx - r * H;
+  // each thread stays on one row (NTH/RB threads per row) and strides its positions: no division
+  constexpr int TPR = NTH / RB;
+  static_assert(NTH % RB == 0, "threads per row");
+  cx<R> *const prow = A + (tidx() / TPR) * 4 * LI::LR;
+  for (int n = tidx() % TPR; n < H; n += TPR) {
     const int e = LI::row(n);
-    cx<R> *b0 = A + (r * 4) * LI::LR + e;
+    cx<R> *b0 = prow + e;
     const cx<R> zu = b0[0], zv = b0[LI::LR], zw = b0[2 * LI::LR];
     if (!VPL) {
       b0[0] = mk<R>(zu.x * zu.x, zu.y * zu.y);                // uu
@@ -177,17 +180,21 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   fft_lds<R, LF, NTH, -1, PLF>(A, twh);
 
   const R sc = (R)(0.5 / (double)MX);
-  for (int idx = threadIdx.x; idx < Kx * RB * NP; idx += NTH) {
-    const int q = idx / Kx, k = idx - q * Kx;
-    const int r = q / NP, p = q - r * NP;
+  {  // each thread stays on one (row, product) pair and strides its modes: no division per output
+    constexpr int SLOTS = RB * NP, TPS = NTH / SLOTS;
+    static_assert(NTH % SLOTS == 0, "threads per output row");
+    const int q = tidx() / TPS, r = q / NP, p = q - r * NP;
     const cx<R> *b0 = A + (r * 4 + p) * LF::LR;
-    const cx<R> zk = b0[LF::row(k)];
-    const cx<R> zm = cconj(b0[LF::row((H - k) % H)]);
-    const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
-    cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
-    x = cscale(x, sc);
-    if (k == 0) x.y = 0;
-    P[((size_t)p * MZ + (z0 + r)) * Kp + k] = x;
+    cx<R> *out = P + ((size_t)p * MZ + (z0 + r)) * Kp;
+    for (int k = tidx() % TPS; k < Kx; k += TPS) {
+      const cx<R> zk = b0[LF::row(k)];
+      const cx<R> zm = cconj(b0[LF::row(k == 0 ? 0 : H - k)]);
+      const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
+      cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
+      x = cscale(x, sc);
+      if (k == 0) x.y = 0;
+      out[k] = x;
+    }
   }
 }
 
