@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/dns_hip.h"
@@ -622,6 +623,35 @@ extern "C" int dns_get_member_scalars(dns_handle *h, int m, double *s) {
 
 extern "C" int dns_get_scalars(dns_handle *h, double *s) { return dns_get_member_scalars(h, 0, s); }
 
+// One history record of record_out (FOU3D.f90:2169-2200) for member m, from the device state and the
+// (0,0) mode of u1 (column 0, rows 0..nyu+1):
+//   rec = flag_ctpress, iter, t, dtv, dtc, dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip
+// err   = max|div| of the last v_corr (error, littleharsh.f90:370-409)
+// Qx    = flowrateIm of u1(:,1) (littleharsh.f90:473-492), summed in the reference's order
+// Umax  = maxvel (littleharsh.f90:550-570): max(0, Re u1(j,1)), j = 0..nyu+1
+// Uslip = (u1(1,1)(-1 - yu(0)) + u1(0,1)(yu(1) + 1)) / (yu(1) - yu(0)), real part (FOU3D.f90:2171)
+extern "C" int dns_history_record(dns_handle *h, int m, double *rec) {
+  if (!h || !rec) return fail(DNS_ERR_ARG, "dns_history_record: null argument");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_history_record: member %d not in 0..%d", m, h->nm - 1);
+  const dnsk::MemberStrides ms = dnsk::member_strides(h->C, h->nyu, h->nyv);
+  const int nr = h->nyu + 2;
+  std::vector<double2> col(nr);
+  DnsState x;
+  HIPCHK(hipMemcpy2DAsync(col.data(), sizeof(double2), h->u1 + m * ms.ru, (size_t)h->C * sizeof(double2),
+                          sizeof(double2), nr, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&x, h->st + m, sizeof x, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double Q = 0.0;
+  for (int j = 1; j <= h->nyv + 1; j++) Q = Q + (h->yv[j] - h->yv[j - 1]) * col[j].x;
+  double Umax = 0.0;
+  for (int j = 0; j <= h->nyu + 1; j++) Umax = std::max(Umax, col[j].x);
+  const double Uslip = (col[1].x * (-1.0 - h->yu[0]) + col[0].x * (h->yu[1] + 1.0)) / (h->yu[1] - h->yu[0]);
+  const double v[13] = {(double)h->prm.flag_ctpress, x.iter, x.t, x.dtv, x.dtc, x.dt, x.divmax, Q, x.QxT,
+                        x.mpgx, x.dgx, Umax, Uslip};
+  memcpy(rec, v, sizeof v);
+  return DNS_OK;
+}
+
 extern "C" int dns_set_dt(dns_handle *h, double dt) {
   if (!h || !(dt > 0)) return fail(DNS_ERR_ARG, "dns_set_dt: bad argument");
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -700,6 +730,9 @@ static int fused_substep(dns_handle *h, int kRK) {
   int rc = plane_stage(h, kRK);
   if (rc) return rc;
   ColArgs a = col_args(h, kRK);
+  // max|div| of this substep's v_corr only, as error(div) sees it (littleharsh.f90:370-409); at kRK = 1
+  // the plane stage's reset has cleared it
+  if (kRK != 1) dnsk::launch_divmax_reset(h->st, h->nm, h->stream);
   if (h->colpass) {
     if (dnsk::launch_column_pass(a, h->stream)) return fail(DNS_ERR_ARG, "column pass: %d rows not instantiated", h->nyu);
     tmark(h, 4);

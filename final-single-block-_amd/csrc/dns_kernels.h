@@ -156,6 +156,7 @@ void launch_widen(const float2 *in, double2 *out, size_t n, hipStream_t st);
 void launch_narrow(const double2 *in, float2 *out, size_t n, hipStream_t st);
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, size_t ms, int nm, double *cfl, hipStream_t st);
 void launch_zero(double *p, int n, hipStream_t st);
+void launch_divmax_reset(DnsState *st, int nm, hipStream_t s);  // every member's max|div| = 0
 int launch_plane_roundtrip(int Mx, int Mz, const double *in, double *phys, double *back, int n,
                            const double2 *twx, const double2 *twh, const double2 *twz, hipStream_t st);
 

@@ -296,6 +296,14 @@ void launch_cfl_reset(DnsState *st, double *slots, int nm, hipStream_t s) {
   cfl_reset_kernel<<<(n + 255) / 256, 256, 0, s>>>(st, slots, nm);
 }
 
+__global__ void divmax_reset_kernel(DnsState *st, int nm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) st[i].divmax = 0.0;
+}
+void launch_divmax_reset(DnsState *st, int nm, hipStream_t s) {
+  divmax_reset_kernel<<<(nm + 255) / 256, 256, 0, s>>>(st, nm);
+}
+
 void launch_zero(double *p, int n, hipStream_t st) { zero_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n); }
 
 int launch_plane_roundtrip(int Mx, int Mz, const double *in, double *phys, double *back, int n,

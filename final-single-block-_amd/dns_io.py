@@ -107,6 +107,51 @@ def write_field(path: str, ff: FieldFile, dummint: np.ndarray | None = None) -> 
         f.write(b"".join(out))
 
 
+# ---- history file hist_NNNNxNNNNxNNNN.dat (start.f90:273-284, FOU3D.f90:2196-2200) -------------------
+# record 1: t, Re, alp, bet, mpgx (f64), nband (i32); record 2: N(4, 0:nband+1) (i32); then per
+# record_out: flag_ctpress, iter (i32), t, dtv, dtc, dt, err, Qx, [QxT,] mpgx, [dgx,] Umax, Uslip (f64),
+# the bracketed pair only when flag_ctpress = 0.
+HIST_FIELDS = {0: ("t", "dtv", "dtc", "dt", "err", "Qx", "QxT", "mpgx", "dgx", "Umax", "Uslip"),
+               1: ("t", "dtv", "dtc", "dt", "err", "Qx", "mpgx", "Umax", "Uslip")}
+
+
+def _frec(payload: bytes) -> bytes:
+    return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+
+def write_hist(path: str, head: dict, N: np.ndarray, records: list, append: bool = False) -> None:
+    """Create (or append to) a history file; records are dicts with flag_ctpress, iter and the fields."""
+    out = []
+    if not append:
+        out.append(_frec(struct.pack("<5di", head["t"], head["Re"], head["alp"], head["bet"], head["mpgx"],
+                                      head.get("nband", 3))))
+        out.append(_frec(np.asarray(N, "<i4").T.tobytes()))
+    for r in records:
+        f = int(r["flag_ctpress"])
+        vals = [float(r[k]) for k in HIST_FIELDS[f]]
+        out.append(_frec(struct.pack("<2i%dd" % len(vals), f, int(r["iter"]), *vals)))
+    with open(path, "ab" if append else "wb") as fh:
+        fh.write(b"".join(out))
+
+
+def read_hist(path: str):
+    """(header dict, N int32[4, nband+2], list of record dicts)."""
+    with open(path, "rb") as fh:
+        recs = list(_records(fh.read()))
+    t, Re, alp, bet, mpgx = struct.unpack_from("<5d", recs[0], 0)
+    (nband,) = struct.unpack_from("<i", recs[0], 40)
+    N = np.frombuffer(recs[1], "<i4").reshape(-1, 4).T.copy()
+    rows = []
+    for r in recs[2:]:
+        f, it = struct.unpack_from("<2i", r, 0)
+        names = HIST_FIELDS[f]
+        vals = struct.unpack_from("<%dd" % len(names), r, 8)
+        d = dict(zip(names, vals))
+        d.update(flag_ctpress=f, iter=it)
+        rows.append(d)
+    return dict(t=t, Re=Re, alp=alp, bet=bet, mpgx=mpgx, nband=nband), N, rows
+
+
 def planes_to_columns(planes: np.ndarray) -> np.ndarray:
     """[nplanes, Nz, Kx] -> column field [ncol, nplanes] (column = k*Kx + i)."""
     npl, nz, kx = planes.shape

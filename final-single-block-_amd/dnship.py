@@ -44,7 +44,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_plane_fft_roundtrip", "dns_modes_to_planes", "dns_planes_to_modes", "dns_sync",
            "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
-           "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path"]
+           "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
+           "dns_history_record"]
 
 
 def lib():
@@ -88,6 +89,7 @@ def lib():
     L.dns_set_plane_variants.argtypes = [vp, _ip]
     L.dns_autotune.argtypes = [vp, ctypes.c_int, _ip, _dp]
     L.dns_set_column_path.argtypes = [vp, ctypes.c_int]
+    L.dns_history_record.argtypes = [vp, ctypes.c_int, _dp]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -258,6 +260,18 @@ class DNS:
         ms = np.zeros(3 * self.NKERN)
         _chk(lib().dns_autotune(self.h, iters, ch.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms)))
         return ch, ms.reshape(self.NKERN, 3)
+
+    HIST_KEYS = ("flag_ctpress", "iter", "t", "dtv", "dtc", "dt", "err", "Qx", "QxT", "mpgx", "dgx", "Umax",
+                 "Uslip")
+
+    def history_record(self, member=0):
+        """record_out's history record (FOU3D.f90:2169-2200) at the current state, as a dict."""
+        r = np.zeros(13)
+        _chk(lib().dns_history_record(self.h, int(member), _p(r)))
+        d = dict(zip(self.HIST_KEYS, r))
+        d["flag_ctpress"] = int(d["flag_ctpress"])
+        d["iter"] = int(d["iter"])
+        return d
 
     def set_column_path(self, path):
         """0: one-pass column kernel where it applies, 1: three sweeps."""

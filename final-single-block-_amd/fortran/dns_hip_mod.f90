@@ -163,6 +163,12 @@ module dns_hip
       type(c_ptr), value :: h, d_planes, d_cols
       integer(c_int), value :: nrows
     end function
+    integer(c_int) function dns_history_record(h, m, rec) bind(C, name='dns_history_record')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: rec(13)
+    end function
     integer(c_int) function dns_set_column_path(h, path) bind(C, name='dns_set_column_path')
       import :: c_int, c_ptr
       type(c_ptr), value :: h

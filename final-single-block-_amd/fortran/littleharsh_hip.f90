@@ -7,8 +7,10 @@
 !
 ! Reads input.in in the reference's positional format (start.f90:86-252) and the initial field files
 ! ``<dirin>/{u1,u2,u3,p}<fnmeb minus its 2-char prefix>`` (start.f90:1545-1997), runs until t >= maxt or
-! nsteps steps, and writes the final fields to ``<dirout>/{u1,u2,u3,p}_NNNNxNNNNxNNNN_tTTTTT.dat`` with the
-! record layout of record_out (FOU3D.f90:2063-2161).
+! nsteps steps.  As record_out (FOU3D.f90:2003-2200) it writes the fields to
+! ``<dirout>/{u1,u2,u3,p}_NNNNxNNNNxNNNN_tTTTTT.dat`` and appends a record to the history file
+! ``<dirout>/hist_NNNNxNNNNxNNNN.dat`` at the start of every nwrite-th step (iwrite, start.f90:1312 and
+! littleharsh.f90:433), and once more at the end of the run.
 
 module lh_state
   use iso_c_binding
@@ -18,7 +20,7 @@ module lh_state
   integer :: kRK = 1, solveU_calls = 0
   integer :: Nspec_x, Nspec_z, Ngal_x, Ngal_z, nyu, nyv, ny, ncol, flag_ctpress, flag_init, nwrite, ppp
   real(8) :: Re, alp, bet, CFL, mpgx, dyq, maxt, t, dt
-  integer :: iter
+  integer :: iter, iwrite
   character(len=120) :: fnameimb, dirin, dirout
 end module
 
@@ -145,10 +147,13 @@ program littleharsh_hip
   end if
   call dns_hip_check(dns_upload_fields(hdns, u1, u2, u3, p, t, int(iter, c_int), 0d0), 'getini')
   iter_start = iter
+  iwrite = iter
   write(*,'(A,I0,A,I0,A,I0,A,ES12.5)') ' littleharsh_hip: ', Nspec_x, ' x ', Nspec_z, ' x ', ny, '  t0 = ', t
+  call open_history()
 
   if (fused) then
     do while (t < maxt .and. iter - iter_start < nmax)
+      if (iter >= iwrite) call record_out_hip(u1, u2, u3, p)
       call dns_hip_check(dns_rk3_step(hdns, 1_c_int), 'rk3_step')
       iter = iter + 1
       call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
@@ -157,6 +162,7 @@ program littleharsh_hip
   else
     ! main loop, littleharsh.f90:163-233
     do while (t < maxt .and. iter - iter_start < nmax)
+      if (iter >= iwrite) call record_out_hip(u1, u2, u3, p)
       do kRK = 1, 3
         call RHS0_u1(du1, u1, Nu1, p, myid)
         call RHS0_u2(du2, u2, Nu2, p, myid)
@@ -177,10 +183,11 @@ program littleharsh_hip
   end if
 
   call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
-  t = sc(1); dt = sc(2); mpgx = sc(4)
-  call dns_hip_check(dns_download_fields(hdns, u1, u2, u3, p), 'finalize')
-  write(*,'(A,I0,A,ES24.16,A,ES24.16,A,ES24.16)') ' iter ', iter, '  t = ', t, '  dt = ', dt, '  Qx = ', sc(5)
-  call write_fields(u1, u2, u3, p)
+  write(*,'(A,I0,A,ES24.16,A,ES24.16,A,ES24.16)') ' iter ', iter, '  t = ', sc(1), '  dt = ', sc(2), &
+    '  Qx = ', sc(5)
+  iwrite = iter
+  call record_out_hip(u1, u2, u3, p)
+  close(30)
   call dns_hip_check(dns_destroy(hdns), 'finalize')
 
 contains
@@ -336,6 +343,51 @@ contains
       write(10) j, gridid, Nspec_x+2, Nspec_z, y(j), buff
     end do
     close(10)
+  end subroutine
+
+  ! history file: opened as start.f90:273-284 (appended to when it exists, else created with the header
+  ! t, Re, alp, bet, mpgx, nband and record_out's N)
+  subroutine open_history()
+    character(len=240) :: path
+    character(len=4) :: e1, e2, e3
+    integer :: N(4, 0:4)
+    logical :: exists
+    write(e1, '(i4.4)') Nspec_x
+    write(e2, '(i4.4)') Nspec_z
+    write(e3, '(i4.4)') ny
+    path = trim(dirout)//'/hist_'//e1//'x'//e2//'x'//e3//'.dat'
+    inquire(file=path, exist=exists)
+    if (exists) then
+      open(30, file=path, form='unformatted', status='old', position='append', action='write')
+    else
+      open(30, file=path, form='unformatted', status='new', action='write')
+      N = 0
+      N(1, 1:3) = Nspec_x
+      N(1, 4) = -2
+      N(2, 1:3) = Nspec_z
+      N(3, 3) = nyv
+      N(4, 3) = nyu
+      write(30) t, Re, alp, bet, mpgx, 3
+      write(30) N
+    end if
+  end subroutine
+
+  ! record_out (FOU3D.f90:2003-2200): field files and one history record (FOU3D.f90:2196-2200)
+  subroutine record_out_hip(u1, u2, u3, p)
+    complex(8), intent(inout) :: u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol)
+    real(8) :: sc(10), h(13)
+    call dns_hip_check(dns_get_scalars(hdns, sc), 'record_out')
+    t = sc(1); dt = sc(2); mpgx = sc(4)
+    call dns_hip_check(dns_download_fields(hdns, u1, u2, u3, p), 'record_out')
+    call write_fields(u1, u2, u3, p)
+    call dns_hip_check(dns_history_record(hdns, 0_c_int, h), 'record_out')
+    if (flag_ctpress == 0) then
+      write(30) flag_ctpress, iter, h(3), h(4), h(5), h(6), h(7), h(8), h(9), h(10), h(11), h(12), h(13)
+    else
+      write(30) flag_ctpress, iter, h(3), h(4), h(5), h(6), h(7), h(8), h(10), h(12), h(13)
+    end if
+    flush(30)
+    iwrite = iwrite + nwrite
   end subroutine
 
   subroutine write_fields(u1, u2, u3, p)

@@ -137,6 +137,14 @@ int dns_modes_to_planes(dns_handle *h, const double *d_cols, int nrows, double *
 /* planes_to_modes_UVP (FOU3D.f90:1694-1824): the inverse gather. */
 int dns_planes_to_modes(dns_handle *h, const double *d_planes, int nrows, double *d_cols);
 
+/* One record of the history file hist_NNNNxNNNNxNNNN.dat that record_out appends (FOU3D.f90:2169-2200,
+ * file opened at start.f90:273-284), for member m, at the current state (a step boundary):
+ *   rec[13] = flag_ctpress, iter, t, dtv, dtc, dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip
+ * err = max|div| of the last v_corr (error, littleharsh.f90:370-409); Qx = flowrateIm of mode (0,0)
+ * (littleharsh.f90:473-492); Umax = maxvel (littleharsh.f90:550-570); Uslip as FOU3D.f90:2171.
+ * The file record holds QxT and dgx only when flag_ctpress = 0 (FOU3D.f90:2196-2200). */
+int dns_history_record(dns_handle *h, int m, double *rec);
+
 /* Column stage of the fused path: path 0 = one kernel per substep (rows split into segments, tridiagonal
  * systems by the partition method, everything read and written once), 1 = three sweeps (thread per
  * column, reference elimination order).  Same equations as RHS0 + nonlinear (column part) + solveU +

@@ -54,17 +54,28 @@ def test_fortran_driver_matches_oracle(tmp_path, mode):
     print("case: fortran driver %s %dx%dx%d %d steps" % (mode, Mx, Mz, ny, n), flush=True)
     r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
-    outs = os.listdir(os.path.join(str(tmp_path), "output"))
-    assert len(outs) == 4, outs
-    otag = outs[0][outs[0].index("_"):]
-    u1, u2, u3, p, head = dns_io.load_state(os.path.join(str(tmp_path), "output"), otag, ny)
+    outdir = os.path.join(str(tmp_path), "output")
+    # record_out at the start of every step (nwrite = 1) and once at the end; files are named by
+    # int(100 t) as in the reference, so snapshots close in time share a file and the newest wins
+    outs = sorted(f for f in os.listdir(outdir) if f.startswith("u1_"))
+    heads = [dns_io.read_field(os.path.join(outdir, f)) for f in outs]
+    last = max(heads, key=lambda hd: hd.iter)
+    otag = [f for f, hd in zip(outs, heads) if hd.iter == n][0][2:]
+    u1, u2, u3, p, head = dns_io.load_state(outdir, otag, ny)
     o = pyoracle.Oracle(Mx, Mz, ny)
     o.set_state(*st)
     o.step(n)
     for a, b in zip((u1, u2, u3, p), o.get_state()):
         assert rel_l2(a, b) <= 1e-12
-    assert head.iter == n
+    assert head.iter == n == last.iter
     assert abs(head.t - o.scalars()["t"]) <= 1e-13
+    hists = [f for f in os.listdir(outdir) if f.startswith("hist_")]
+    assert hists == ["hist_%04dx%04dx%04d.dat" % (2 * Mx // 3, 2 * Mz // 3, ny)]
+    hh, N, rows = dns_io.read_hist(os.path.join(outdir, hists[0]))
+    assert [r["iter"] for r in rows] == list(range(n + 1))
+    assert rows[0]["t"] == 0.0 and abs(rows[-1]["t"] - o.scalars()["t"]) <= 1e-13
+    assert abs(rows[-1]["Qx"] - o.scalars()["Qx"]) <= 1e-13 * abs(o.scalars()["Qx"])
+    assert all(r["flag_ctpress"] == 1 for r in rows) and hh["Re"] == 3250.0
 
 
 def test_fortran_driver_rejects_bad_size(tmp_path):

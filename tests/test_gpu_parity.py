@@ -342,6 +342,64 @@ def test_column_paths_agree(prec, Mx, Mz, ny, ctp):
     assert sa["divmax"] < 1e-9 and sb["divmax"] < 1e-9
 
 
+@pytest.mark.parametrize("ctp", [1, 0])
+def test_history_record_vs_oracle(ctp):
+    """record_out's history record (FOU3D.f90:2169-2200) after 3 steps, against the oracle's state: the
+    scalars the oracle keeps, err = max|div| of its last v_corr (error, littleharsh.f90:370-409), and
+    maxvel / Uslip of its mode (0,0) (littleharsh.f90:550-570, FOU3D.f90:2171)."""
+    Mx, Mz, ny, n = 48, 48, 33, 3
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=13)
+    print("case: history record ctp=%d" % ctp, flush=True)
+    g = dnship.DNS(Mx, Mz, ny, flag_ctpress=ctp)
+    g.upload(*st)
+    g.step(n)
+    h = g.history_record()
+    g.close()
+    o = pyoracle.Oracle(Mx, Mz, ny, flag_ctpress=ctp)
+    o.set_state(*st)
+    o.step(n)
+    sc, su = o.scalars(), o.setup()
+    u1 = o.get_state()[0]
+    _, div = o.get_psi()
+    assert h["flag_ctpress"] == ctp and h["iter"] == n
+    for k in ("t", "dt", "dtc", "mpgx"):
+        assert abs(h[k] - sc[k]) <= 1e-13 * abs(sc[k]), k
+    # record_out recomputes flowrateIm from the current u1 (littleharsh.f90:473-492); with a constant
+    # pressure gradient that is the oracle's own Qx of meanflow_ctP
+    yv = su["yv"]
+    qx = sum((yv[j] - yv[j - 1]) * u1[0, j].real for j in range(1, ny))
+    if ctp == 1:
+        assert abs(qx - sc["Qx"]) <= 1e-14 * abs(qx)
+    assert abs(h["Qx"] - qx) <= 1e-13 * abs(qx)
+    assert abs(h["dtv"] - su["dtv"]) <= 1e-15 * su["dtv"]
+    if ctp == 0:
+        assert abs(h["QxT"] - sc["QxT"]) <= 1e-13 * abs(sc["QxT"]) and abs(h["dgx"] - sc["dgx"]) <= 1e-12
+    assert h["err"] < 1e-9 and abs(h["err"] - np.abs(div).max()) <= 1e-12
+    c0 = u1[0].real                       # mode (0,0), rows 0..nyu+1
+    yu = su["yu"]
+    assert abs(h["Umax"] - max(0.0, c0.max())) <= 1e-13 * max(1.0, c0.max())
+    uslip = (c0[1] * (-1.0 - yu[0]) + c0[0] * (yu[1] + 1.0)) / (yu[1] - yu[0])
+    assert abs(h["Uslip"] - uslip) <= 1e-14 * max(1.0, abs(uslip))
+
+
+def test_history_record_linear_profile():
+    """Uslip extrapolates the (0,0) mode linearly to the wall y = -1 from the ghost row and row 1: for
+    u = a + b y it is exactly a - b; Umax is the largest real part (never below 0)."""
+    Mx, Mz, ny = 24, 24, 17
+    g = dnship.DNS(Mx, Mz, ny)
+    u1, u2, u3, p = dns_ic.seeded_channel(Mx, Mz, ny, seed=2)
+    yu = dnship.ygrid(ny)["yu"]
+    a, b = 0.75, 0.5
+    u1 = u1.copy()
+    u1[0] = a + b * yu
+    g.upload(u1, u2, u3, p)
+    h = g.history_record()
+    g.close()
+    assert abs(h["Uslip"] - (a - b)) <= 1e-14
+    assert h["Umax"] == (a + b * yu).max()
+    assert h["iter"] == 0 and h["t"] == 0.0
+
+
 def test_restart_bit_exact():
     """Checkpoint/restart at a step boundary (record_out/getini): download the fields, write and read
     the reference's field files, upload into a fresh handle and continue; the result equals the
