@@ -45,6 +45,8 @@ struct dns_handle {
   // variant per kernel: z-inverse, x-pass u/v, z-forward u/v, column sweeps up/down/final
   int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool colpass = false;  // column stage: one-pass kernel (true) or the three sweeps
+  double *stat_acc = nullptr, *stat_sp = nullptr, *stat_part = nullptr;  // statistics (allocated on first use)
+  int istat = 0;
   size_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -916,6 +918,83 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   drop_graph(h);
   if (rc == DNS_OK) HIPCHK(hipGetLastError());
   return rc;
+}
+
+// ---- statistics and spectra (stats.f90, spectra.f90) ---------------------------------------------
+static size_t stats_sp_per_member(const dns_handle *h) {
+  const size_t C = h->C, nu = (size_t)(h->nyu + 2) * C, nv = (size_t)(h->nyv + 2) * C;
+  return 3 * nu + nv + (size_t)h->nyu * C;
+}
+
+extern "C" int dns_accumulate_stats(dns_handle *h) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_accumulate_stats: null handle");
+  for (int m = 0; m < h->nm; m++)
+    if (!h->uploaded[m]) return fail(DNS_ERR_STATE, "dns_accumulate_stats: no fields uploaded for member %d", m);
+  const int nchunk = dnsk::stats_nchunk(h->C);
+  if (!h->stat_acc) {
+    int rc = 0;
+    rc |= dalloc(h, &h->stat_acc, (size_t)h->nm * dnsk::STAT_NQ * (h->nyu + 2));
+    rc |= dalloc(h, &h->stat_sp, (size_t)h->nm * stats_sp_per_member(h));
+    rc |= dalloc(h, &h->stat_part, (size_t)h->nm * (h->nyu + 2) * nchunk * dnsk::STAT_NQ);
+    if (rc) return fail(DNS_ERR_HIP, "dns_accumulate_stats: out of device memory");
+  }
+  dnsk::StatsArgs a;
+  a.Kx = h->Kx; a.Nx = h->Nx; a.Nz = h->Nz; a.Mx = h->Mx; a.Mz = h->Mz; a.C = h->C;
+  a.nyu = h->nyu; a.nyv = h->nyv; a.nm = h->nm; a.nchunk = nchunk;
+  a.bet = h->prm.bet; a.gi1 = h->gi1; a.gi2 = h->gi2;
+  a.yu = h->d_yu; a.yv = h->d_yv; a.dthdyv = h->d_dthdyv;
+  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p;
+  a.partial = h->stat_part; a.acc = h->stat_acc; a.sp = h->stat_sp;
+  dnsk::launch_stats(a, h->stream);
+  HIPCHK(hipGetLastError());
+  h->istat++;
+  return DNS_OK;
+}
+
+extern "C" int dns_reset_stats(dns_handle *h) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_reset_stats: null handle");
+  if (h->stat_acc) {
+    HIPCHK(hipMemsetAsync(h->stat_acc, 0, (size_t)h->nm * dnsk::STAT_NQ * (h->nyu + 2) * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(h->stat_sp, 0, (size_t)h->nm * stats_sp_per_member(h) * sizeof(double), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  h->istat = 0;
+  return DNS_OK;
+}
+
+extern "C" int dns_get_stats(dns_handle *h, int m, double *stats, double *spectra, int *istat) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_get_stats: null handle");
+  if (m < 0 || m >= h->nm) return fail(DNS_ERR_ARG, "dns_get_stats: member %d not in 0..%d", m, h->nm - 1);
+  const size_t nj = h->nyu + 2;
+  if (istat) *istat = h->istat;
+  if (stats) {
+    if (h->stat_acc) {
+      HIPCHK(hipMemcpyAsync(stats, h->stat_acc + (size_t)m * dnsk::STAT_NQ * nj, dnsk::STAT_NQ * nj * sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+    } else {
+      memset(stats, 0, dnsk::STAT_NQ * nj * sizeof(double));
+    }
+  }
+  if (spectra) {
+    const size_t per = stats_sp_per_member(h), C = h->C;
+    std::vector<double> dev(per, 0.0);
+    if (h->stat_sp) {
+      HIPCHK(hipMemcpyAsync(dev.data(), h->stat_sp + (size_t)m * per, per * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    // device rows x columns -> the reference's sp(j, column) (j fastest), array by array
+    const int rows[5] = {h->nyu + 2, h->nyv + 2, h->nyu + 2, h->nyu + 2, h->nyu};
+    size_t off = 0;
+    for (int k = 0; k < 5; k++) {
+      const int nr = rows[k];
+      for (size_t c = 0; c < C; c++)
+        for (int j = 0; j < nr; j++) spectra[off + c * nr + j] = dev[off + (size_t)j * C + c];
+      off += (size_t)nr * C;
+    }
+  }
+  return DNS_OK;
 }
 
 extern "C" int dns_set_column_path(dns_handle *h, int path) {

@@ -172,6 +172,22 @@ void launch_qx(const ColArgs &a, hipStream_t st);
 void launch_fused_up(const ColArgs &a, int variant, hipStream_t st);  // 0 thread/column, 1-2 wave/solve
 void launch_fused_down(const ColArgs &a, int variant, hipStream_t st);   // 0 thread/column, 1-2 four waves
 void launch_fused_final(const ColArgs &a, int variant, hipStream_t st);  // 0 thread/column, 1 + prefetch, 2 four waves
+// statistics and spectra (stats_kernels.hip)
+constexpr int STAT_NQ = 13;       // Um U2m Wm W2m UVm UWm Vm V2m VWm wxm wx2m Pm P2m
+
+struct StatsArgs {
+  int Kx, Nx, Nz, Mx, Mz, C, nyu, nyv, nm, nchunk;
+  double bet, gi1, gi2;
+  const double *yu, *yv, *dthdyv;
+  const double2 *u1, *u2, *u3, *p;
+  double *partial;  // [m][j][chunk][q]
+  double *acc;      // [m][q][j], j = 0..nyu+1
+  double *sp;       // spectra [m]: spU (nyu+2 rows), spV (nyv+2), spW (nyu+2), spUV (nyu+2), spP (nyu), rows x C
+};
+
+int stats_nchunk(int C);
+void launch_stats(const StatsArgs &a, hipStream_t st);
+
 // one-pass column stage (column_pass.hip): flag_ctpress = 1, 8 <= nyu <= 512
 bool column_pass_supported(int nyu);
 int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered

@@ -152,6 +152,58 @@ def read_hist(path: str):
     return dict(t=t, Re=Re, alp=alp, bet=bet, mpgx=mpgx, nband=nband), N, rows
 
 
+# ---- statistics and spectra files (stats.f90:340-362, spectra.f90:167-186) ----------------------------
+# stats_NNNNxNNNNxNNNN_TTTTT.dat: record 1 as a field file (t, Re, alp, bet, mpgx, nband, iter,
+# dummint(88)); record 2: the physical grid sizes in N's layout (the reference names it Ngal); records
+# 3, 4: yu, dthetai, dthdyu and yv, dthetai, dthdyv; record 5: istat (i32); then one f64 record per
+# statistic in STATS_FILE_ORDER over its grid's planes (u: 0..nyu+1, v: 0..nyv+1, p: 1..nyu).
+# sp{U,V,W,UV,P}_...: the same first five records (N of the field files in record 2), then buffSP.
+STATS_FILE_ORDER = ("Um", "U2m", "Vm", "V2m", "Wm", "W2m", "Pm", "P2m", "UVm", "UWm", "VWm", "wxm", "wx2m")
+_STAT_GRID = {"Vm": "v", "V2m": "v", "VWm": "v", "wxm": "v", "wx2m": "v", "Pm": "p", "P2m": "p"}
+
+
+def _stats_head(recs):
+    t, Re, alp, bet, mpgx = struct.unpack_from("<5d", recs[0], 0)
+    nband, it = struct.unpack_from("<2i", recs[0], 40)
+    N = np.frombuffer(recs[1], "<i4").reshape(-1, 4).T.copy()
+    (istat,) = struct.unpack("<i", recs[4])
+    return dict(t=t, Re=Re, alp=alp, bet=bet, mpgx=mpgx, nband=nband, iter=it, N=N, istat=istat)
+
+
+def read_stats(path: str) -> dict:
+    """Header fields, istat and the 13 statistics (arrays over the planes written)."""
+    with open(path, "rb") as fh:
+        recs = list(_records(fh.read()))
+    out = _stats_head(recs)
+    for name, r in zip(STATS_FILE_ORDER, recs[5:]):
+        out[name] = np.frombuffer(r, "<f8").copy()
+    return out
+
+
+def fold_spectrum(sp: np.ndarray, Nx: int, Nz: int, j0: int = 0) -> np.ndarray:
+    """recvwrspec's buffSP(0:Nx/2, 1:Nz/2+1, rows) from sp[column, row] (spectra.f90:142-165): column
+    (i, k) goes to (i, kk), kk = k for k <= Nz/2 and Nz+2-k above (k 1-based), +kz and -kz adding; entries
+    i = 1..Nx/2, kk = 1..Nz/2 doubled.  Returned as [rows, Nz/2+1, Nx/2+1] (the file's order reversed)."""
+    Kx = Nx // 2 + 1
+    ncol, nr = sp.shape
+    buff = np.zeros((nr, Nz // 2 + 1, Kx))
+    for col in range(ncol):
+        i, k = col % Kx, col // Kx + 1
+        kk = k if k <= Nz // 2 else Nz + 2 - k
+        buff[:, kk - 1, i] += sp[col]
+    buff[:, :Nz // 2, 1:] *= 2.0
+    return buff
+
+
+def read_spectrum(path: str, Nx: int, Nz: int) -> dict:
+    """Header fields, istat and buffSP as [rows, Nz/2+1, Nx/2+1]."""
+    with open(path, "rb") as fh:
+        recs = list(_records(fh.read()))
+    out = _stats_head(recs)
+    out["sp"] = np.frombuffer(recs[5], "<f8").reshape(-1, Nz // 2 + 1, Nx // 2 + 1).copy()
+    return out
+
+
 def planes_to_columns(planes: np.ndarray) -> np.ndarray:
     """[nplanes, Nz, Kx] -> column field [ncol, nplanes] (column = k*Kx + i)."""
     npl, nz, kx = planes.shape

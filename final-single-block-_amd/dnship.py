@@ -45,7 +45,7 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
            "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
-           "dns_history_record"]
+           "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats"]
 
 
 def lib():
@@ -90,6 +90,9 @@ def lib():
     L.dns_autotune.argtypes = [vp, ctypes.c_int, _ip, _dp]
     L.dns_set_column_path.argtypes = [vp, ctypes.c_int]
     L.dns_history_record.argtypes = [vp, ctypes.c_int, _dp]
+    L.dns_accumulate_stats.argtypes = [vp]
+    L.dns_reset_stats.argtypes = [vp]
+    L.dns_get_stats.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -272,6 +275,35 @@ class DNS:
         d["flag_ctpress"] = int(d["flag_ctpress"])
         d["iter"] = int(d["iter"])
         return d
+
+    STATS = ("Um", "U2m", "Wm", "W2m", "UVm", "UWm", "Vm", "V2m", "VWm", "wxm", "wx2m", "Pm", "P2m")
+
+    def accumulate_stats(self):
+        """Adds the statistics and spectra of the current state (stats.f90, spectra.f90)."""
+        _chk(lib().dns_accumulate_stats(self.h))
+
+    def reset_stats(self):
+        _chk(lib().dns_reset_stats(self.h))
+
+    def stats(self, member=0, spectra=False):
+        """dict: istat, the 13 plane statistics (arrays of nyu+2; Pm, P2m at index j = 1..nyu+1) and, with
+        spectra=True, spU, spV, spW, spUV, spP as [ncol, rows] (the reference's sp(j, column))."""
+        nj = self.nyu + 2
+        st = np.zeros(13 * nj)
+        shapes = [(self.C, self.nyu + 2), (self.C, self.nyv + 2), (self.C, self.nyu + 2), (self.C, self.nyu + 2),
+                  (self.C, self.nyu)]
+        sp = np.zeros(sum(a * b for a, b in shapes)) if spectra else None
+        n = ctypes.c_int()
+        _chk(lib().dns_get_stats(self.h, int(member), _p(st), _p(sp) if spectra else None, ctypes.byref(n)))
+        out = {"istat": n.value}
+        for q, name in enumerate(self.STATS):
+            out[name] = st[q * nj:(q + 1) * nj].copy()
+        if spectra:
+            off = 0
+            for name, shp in zip(("spU", "spV", "spW", "spUV", "spP"), shapes):
+                out[name] = sp[off:off + shp[0] * shp[1]].reshape(shp).copy()
+                off += shp[0] * shp[1]
+        return out
 
     def set_column_path(self, path):
         """0: one-pass column kernel where it applies, 1: three sweeps."""

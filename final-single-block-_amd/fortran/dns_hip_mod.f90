@@ -169,6 +169,23 @@ module dns_hip
       integer(c_int), value :: m
       real(c_double), intent(out) :: rec(13)
     end function
+    integer(c_int) function dns_accumulate_stats(h) bind(C, name='dns_accumulate_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dns_reset_stats(h) bind(C, name='dns_reset_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    ! stats(nyu+2, 13) in stats.f90's order; spectra: spU, spV, spW, spUV, spP one after another, each
+    ! sp(rows, columns) as in spectra.f90 (rows nyu+2, nyv+2, nyu+2, nyu+2, nyu); c_null_ptr skips either
+    integer(c_int) function dns_get_stats(h, m, stats, spectra, istat) bind(C, name='dns_get_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: m
+      type(c_ptr), value :: stats, spectra
+      integer(c_int), intent(out) :: istat
+    end function
     integer(c_int) function dns_set_column_path(h, path) bind(C, name='dns_set_column_path')
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -10,7 +10,10 @@
 ! nsteps steps.  As record_out (FOU3D.f90:2003-2200) it writes the fields to
 ! ``<dirout>/{u1,u2,u3,p}_NNNNxNNNNxNNNN_tTTTTT.dat`` and appends a record to the history file
 ! ``<dirout>/hist_NNNNxNNNNxNNNN.dat`` at the start of every nwrite-th step (iwrite, start.f90:1312 and
-! littleharsh.f90:433), and once more at the end of the run.
+! littleharsh.f90:433), and once more at the end of the run.  Every nstat = min(20, nwrite) steps
+! (start.f90:410, FOU3D.f90:77-81) the plane statistics and spectra are accumulated on the device; at each
+! write step they go to ``<dirout>/stats_NNNNxNNNNxNNNN_TTTTT.dat`` (stats.f90:244-413) and
+! ``<dirout>/sp{U,V,W,UV,P}_NNNNxNNNNxNNNN_TTTTT.dat`` (spectra.f90:71-188) and are reset.
 
 module lh_state
   use iso_c_binding
@@ -20,7 +23,7 @@ module lh_state
   integer :: kRK = 1, solveU_calls = 0
   integer :: Nspec_x, Nspec_z, Ngal_x, Ngal_z, nyu, nyv, ny, ncol, flag_ctpress, flag_init, nwrite, ppp
   real(8) :: Re, alp, bet, CFL, mpgx, dyq, maxt, t, dt
-  integer :: iter, iwrite
+  integer :: iter, iwrite, iter0, nstat
   character(len=120) :: fnameimb, dirin, dirout
 end module
 
@@ -148,12 +151,14 @@ program littleharsh_hip
   call dns_hip_check(dns_upload_fields(hdns, u1, u2, u3, p, t, int(iter, c_int), 0d0), 'getini')
   iter_start = iter
   iwrite = iter
+  iter0 = iter
+  nstat = min(20, nwrite)
   write(*,'(A,I0,A,I0,A,I0,A,ES12.5)') ' littleharsh_hip: ', Nspec_x, ' x ', Nspec_z, ' x ', ny, '  t0 = ', t
   call open_history()
 
   if (fused) then
     do while (t < maxt .and. iter - iter_start < nmax)
-      if (iter >= iwrite) call record_out_hip(u1, u2, u3, p)
+      call step_output(u1, u2, u3, p)
       call dns_hip_check(dns_rk3_step(hdns, 1_c_int), 'rk3_step')
       iter = iter + 1
       call dns_hip_check(dns_get_scalars(hdns, sc), 'scalars')
@@ -162,7 +167,7 @@ program littleharsh_hip
   else
     ! main loop, littleharsh.f90:163-233
     do while (t < maxt .and. iter - iter_start < nmax)
-      if (iter >= iwrite) call record_out_hip(u1, u2, u3, p)
+      call step_output(u1, u2, u3, p)
       do kRK = 1, 3
         call RHS0_u1(du1, u1, Nu1, p, myid)
         call RHS0_u2(du2, u2, Nu2, p, myid)
@@ -370,6 +375,121 @@ contains
       write(30) t, Re, alp, bet, mpgx, 3
       write(30) N
     end if
+  end subroutine
+
+  ! the kRK = 1 output of FOU3D.f90:77-90, 138-154, 338-348: statistics every nstat steps; at write steps
+  ! record_out, then the statistics and spectra files, after which the sums start again
+  subroutine step_output(u1, u2, u3, p)
+    complex(8), intent(inout) :: u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol)
+    if (iter - iter0 >= nstat) then
+      iter0 = iter
+      call dns_hip_check(dns_accumulate_stats(hdns), 'stats')
+    end if
+    if (iter >= iwrite) then
+      call record_out_hip(u1, u2, u3, p)
+      call write_stats_files()
+      call dns_hip_check(dns_reset_stats(hdns), 'write_stats')
+    end if
+  end subroutine
+
+  ! stats.f90:340-362 and spectra.f90:101-188 for one process holding every column
+  subroutine write_stats_files()
+    real(8), allocatable, target :: st(:,:), sp(:)
+    real(8), allocatable :: yu(:), yv(:), du(:), dv(:)
+    integer(c_int), target :: istat
+    integer :: N(4, 0:4), dummint(88), q, off
+    integer, parameter :: qorder(13) = [1, 2, 7, 8, 3, 4, 12, 13, 5, 6, 9, 10, 11]
+    character(len=240) :: tag
+    allocate(st(0:nyu+1, 13), sp(ncol * (4*nyu + nyv + 8)))
+    call dns_hip_check(dns_get_stats(hdns, 0_c_int, c_loc(st), c_loc(sp), istat), 'write_stats')
+    allocate(yu(0:ny), yv(0:ny-1), du(0:ny), dv(0:ny-1))
+    call dns_hip_check(dns_ygrid(int(ny, c_int), dyq, int(ppp, c_int), yu, yv, du, dv), 'ygrid')
+    tag = file_tag()
+    call header_n(N)
+    dummint = 0
+    open(10, file=trim(dirout)//'/stats_'//trim(tag)//'.dat', form='unformatted', status='replace')
+    write(10) t, Re, alp, bet, mpgx, 3, iter, dummint
+    N(1, 1:3) = Ngal_x   ! the physical grid (Ngal) in place of the spectral sizes
+    N(2, 1:3) = Ngal_z
+    write(10) N
+    write(10) yu, 1d0, du
+    write(10) yv, 1d0, dv
+    write(10) int(istat)
+    do q = 1, 13
+      select case (qorder(q))
+      case (7, 8, 9, 10, 11)          ! v grid: planes 0..nyv+1
+        write(10) st(0:nyv+1, qorder(q))
+      case (12, 13)                   ! p grid: planes 1..nyu
+        write(10) st(1:nyu, qorder(q))
+      case default                    ! u grid: planes 0..nyu+1
+        write(10) st(0:nyu+1, qorder(q))
+      end select
+    end do
+    close(10)
+    call header_n(N)
+    off = 0
+    call write_spectrum('spU', nyu + 2, 0, sp, off, tag, N, yu, du, yv, dv, int(istat))
+    call write_spectrum('spV', nyv + 2, 0, sp, off, tag, N, yu, du, yv, dv, int(istat))
+    call write_spectrum('spW', nyu + 2, 0, sp, off, tag, N, yu, du, yv, dv, int(istat))
+    call write_spectrum('spUV', nyu + 2, 0, sp, off, tag, N, yu, du, yv, dv, int(istat))
+    call write_spectrum('spP', nyu, 1, sp, off, tag, N, yu, du, yv, dv, int(istat))
+  end subroutine
+
+  ! recvwrspec (spectra.f90:125-188) for rows j0..j0+nr-1 of one spectrum (sp(off+1:) holds sp(j, column)):
+  ! the column (i, k) goes to (i, kk) with kk = k for k <= Nz/2 and Nz+2-k above, +kz and -kz adding;
+  ! entries 1..Nx/2 x 1..Nz/2 are doubled
+  subroutine write_spectrum(name, nr, j0, sp, off, tag, N, yu, du, yv, dv, istat)
+    character(*), intent(in) :: name, tag
+    integer, intent(in) :: nr, j0, N(4, 0:4), istat
+    integer, intent(inout) :: off
+    real(8), intent(in) :: sp(:), yu(:), du(:), yv(:), dv(:)
+    real(8), allocatable :: buff(:,:,:)
+    integer :: col, i, k, kk, j, dummint(88)
+    allocate(buff(0:Nspec_x/2, 1:Nspec_z/2+1, j0:j0+nr-1))
+    buff = 0d0
+    do col = 1, ncol
+      i = mod(col - 1, Nspec_x/2 + 1)
+      k = (col - 1) / (Nspec_x/2 + 1) + 1
+      kk = k
+      if (k > Nspec_z/2) kk = Nspec_z + 2 - k
+      do j = 0, nr - 1
+        buff(i, kk, j0 + j) = buff(i, kk, j0 + j) + sp(off + (col - 1) * nr + j + 1)
+      end do
+    end do
+    buff(1:Nspec_x/2, 1:Nspec_z/2, :) = 2d0 * buff(1:Nspec_x/2, 1:Nspec_z/2, :)
+    dummint = 0
+    open(10, file=trim(dirout)//'/'//name//'_'//trim(tag)//'.dat', form='unformatted', status='replace')
+    write(10) t, Re, alp, bet, mpgx, 3, iter, dummint
+    write(10) N
+    write(10) yu, 1d0, du
+    write(10) yv, 1d0, dv
+    write(10) istat
+    write(10) buff
+    close(10)
+    off = off + nr * ncol
+  end subroutine
+
+  ! NNNNxNNNNxNNNN_TTTTT of the statistics files: the sizes and int(100 t) of record_out
+  ! (start.f90:263-265, FOU3D.f90:2063)
+  function file_tag() result(tag)
+    character(len=240) :: tag
+    character(len=4) :: e1, e2, e3
+    character(len=5) :: e4
+    write(e1, '(i4.4)') Nspec_x
+    write(e2, '(i4.4)') Nspec_z
+    write(e3, '(i4.4)') ny
+    write(e4, '(i5.5)') int(100d0*t)
+    tag = e1//'x'//e2//'x'//e3//'_'//e4
+  end function
+
+  subroutine header_n(N)
+    integer, intent(out) :: N(4, 0:4)
+    N = 0
+    N(1, 1:3) = Nspec_x
+    N(1, 4) = -2
+    N(2, 1:3) = Nspec_z
+    N(3, 3) = nyv
+    N(4, 3) = nyu
   end subroutine
 
   ! record_out (FOU3D.f90:2003-2200): field files and one history record (FOU3D.f90:2196-2200)

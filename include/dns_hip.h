@@ -145,6 +145,20 @@ int dns_planes_to_modes(dns_handle *h, const double *d_planes, int nrows, double
  * The file record holds QxT and dgx only when flag_ctpress = 0 (FOU3D.f90:2196-2200). */
 int dns_history_record(dns_handle *h, int m, double *rec);
 
+/* Statistics and spectra (stats.f90:1-97, spectra.f90:1-60, omega_x of FOU3D.f90:307-338), accumulated
+ * on the device at the current state (the reference accumulates them at kRK = 1 of every nstat-th step,
+ * from the fields at the start of that step: call dns_accumulate_stats before dns_rk3_step).
+ * dns_get_stats: istat (number of accumulations), stats[13*(nyu+2)] = Um, U2m, Wm, W2m, UVm, UWm (u
+ * planes 0..nyu+1), Vm, V2m, VWm, wxm, wx2m (v planes 0..nyu+1), Pm, P2m (p planes 1..nyu+1 at index j,
+ * index 0 unused), each array nyu+2 long; spectra (may be NULL) = spU [ncol][nyu+2], spV [ncol][nyv+2],
+ * spW, spUV [ncol][nyu+2], spP [ncol][nyu] in the reference's sp(j, column) layout.  Pm, P2m sum the
+ * spectral p planes that record_out leaves in ppPL, i.e. the values of a statistics step that is also a
+ * write step.  The physical-plane sums are formed with the exact discrete sum rules of the reference's
+ * transforms (agreement with the reference to rounding, tested on its own output). */
+int dns_accumulate_stats(dns_handle *h);
+int dns_get_stats(dns_handle *h, int m, double *stats, double *spectra, int *istat);
+int dns_reset_stats(dns_handle *h);
+
 /* Column stage of the fused path: path 0 = one kernel per substep (rows split into segments, tridiagonal
  * systems by the partition method, everything read and written once), 1 = three sweeps (thread per
  * column, reference elimination order).  Same equations as RHS0 + nonlinear (column part) + solveU +

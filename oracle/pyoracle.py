@@ -182,6 +182,11 @@ def time_reference_substeps(Mx, Mz, ny, u1, u2, u3, p, nsub=1, timeout=1800, **k
     return run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=4, nsteps=nsub, timeout=timeout, **kw)
 
 
+# plane statistics of stats.f90 in the order the harness writes them (mode 5): u-grid planes 0..nyu+1
+# (U, W, UV, UW), v-grid planes 0..nyu+1 (V, VW, omega_x), p-grid planes 1..nyu+1 (P)
+STATS_NAMES = ("Um", "U2m", "Wm", "W2m", "UVm", "UWm", "Vm", "V2m", "VWm", "wxm", "wx2m", "Pm", "P2m")
+
+
 def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=0.0, it=0,
                   Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0, ppp=5,
                   flag_ctpress=1, QxT=0.0, Nu=None, fft_planes=None, timeout=3600):
@@ -190,6 +195,8 @@ def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=
     mode 1: ``nsteps`` full RK3 steps -> dict(u1,u2,u3,p,Nu1,Nu2,Nu3,t,dt,mpgx,Qx,dgx,hist)
     mode 2: one substep ``kRK`` with the stage-by-stage arrays (``dt`` is used when kRK > 1)
     mode 3: ``fft_planes`` [n, Mz, Mx+2] through four_to_phys_du / phys_to_four_du
+    mode 5: statistics and spectra of the state (stats.f90, spectra.f90) -> dict of STATS_NAMES arrays,
+            spU, spV, spW, spUV, spP [ncol, rows] and istat
     """
     if not have_reference():
         raise FileNotFoundError(REF_HARNESS)
@@ -216,6 +223,23 @@ def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=
         if mode == 4:
             with open(os.path.join(d, "timing.txt")) as f:
                 return float(f.read().split()[1])
+        if mode == 5:
+            Nx, Nz = (2 * Mx) // 3, (2 * Mz) // 3
+            C = (Nx // 2 + 1) * Nz
+            nyu = ny - 1
+            raw = np.fromfile(os.path.join(d, "stats_out.bin"), dtype=np.uint8)
+            res = {"istat": int(raw[:4].view(np.int32)[0])}
+            vals = raw[4:].view(np.float64)
+            off = 0
+            for name in STATS_NAMES:
+                n = nyu + 1 if name in ("Pm", "P2m") else nyu + 2
+                res[name] = vals[off:off + n].copy()
+                off += n
+            for name, nr in (("spU", nyu + 2), ("spV", nyu + 1), ("spW", nyu + 2), ("spUV", nyu + 2),
+                             ("spP", nyu)):
+                res[name] = vals[off:off + C * nr].reshape(C, nr).copy()
+                off += C * nr
+            return res
         if mode == 3:
             out = np.fromfile(os.path.join(d, "fft_out.bin"), dtype=np.float64)
             n = fft_planes.shape[0]
@@ -272,3 +296,86 @@ def run_reference(Mx, Mz, ny, u1, u2, u3, p, mode=1, nsteps=1, kRK=1, dt=0.0, t=
         setup["gbc"] = sb[o + 5]
         res["setup"] = setup
         return res
+
+
+# ------------------------------------------------------------------------------------------------
+def plane_stats(Mx, Mz, ny, u1, u2, u3, p, yu, yv, dthdyv, bet):
+    """One accumulation of stats.f90:1-97 (with omega_x of FOU3D.f90:307-338) from the spectral state,
+    by the discrete sum rules of the reference's transforms: a physical-space plane sum of the 3/2
+    grid (Mx x Mz points) is Mx Mz Re u(0, 0); a sum of products is Mx Mz Q(u, v) with
+    Q = sum_{kx>=1} 2 Re(u v*) + sum_{kx=0} (Re(u_kz v_kz*) + Re(u_kz v_-kz)) / 2 (the real synthesis
+    of rft_buff.f:76-84 ignores Im of the kx = 0 coefficient; kz = -Nz/2 has no partner on the padded
+    grid).  omega_x is summed on the Nx x Nz grid of four_to_phys_N, where kx = Nx/2 also takes the
+    half-terms and kz = -Nz/2 pairs with itself.  Planes the reference keeps in spectral layout (u
+    planes 0 and nyu+1, v plane 0, p planes) sum the interleaved (Re, Im) entries.  Fields [C, rows]."""
+    Nx, Nz = 2 * (Mx // 3), 2 * (Mz // 3)
+    Kx = Nx // 2 + 1
+    nyu, nyv = ny - 1, ny - 2
+    C = Kx * Nz
+    ii = np.arange(C) % Kx
+    kk = np.arange(C) // Kx
+    kz = np.where(kk < Nz // 2, kk, kk - Nz) * bet
+    k0 = np.arange(Nz) * Kx                      # kx = 0 columns
+    kn = k0 + Nx // 2                            # kx = Nx/2 columns
+    pk = np.where(np.arange(Nz) == 0, 0, Nz - np.arange(Nz))
+    pad_ok = np.arange(Nz) != Nz // 2
+
+    def Qp(a, b):
+        s = 2.0 * np.sum((a[ii >= 1] * np.conj(b[ii >= 1])).real)
+        t = (a[k0] * np.conj(b[k0])).real + np.where(pad_ok, (a[k0] * b[pk * Kx]).real, 0.0)
+        return Mx * Mz * (s + 0.5 * np.sum(t))
+
+    def Qn(w):
+        mid = (ii >= 1) & (ii < Nx // 2)
+        s = 2.0 * np.sum(np.abs(w[mid]) ** 2)
+        for cols in (k0, kn):
+            s += 0.5 * np.sum(np.abs(w[cols]) ** 2 + (w[cols] * w[pk * Kx + (cols[0] % Kx)]).real)
+        return Nx * Nz * s
+
+    def ssum(a): return float(np.sum(a.real + a.imag))
+    def sdot(a, b): return float(np.sum(a.real * b.real + a.imag * b.imag))
+
+    gi1 = (yu[0] - yu[1]) / (yv[0] - yu[1])
+    gi2 = (yu[nyu + 1] - yu[nyu]) / (yv[nyv + 1] - yu[nyu])
+    u2i = np.zeros((C, nyu + 2), complex)          # interp_v (FOU3D.f90:396-418)
+    for j in range(1, nyu + 1):
+        u2i[:, j] = ((yu[j] - yv[j - 1]) * u2[:, j] + (yv[j] - yu[j]) * u2[:, j - 1]) / (yv[j] - yv[j - 1])
+    u2i[:, 0] = gi1 * (u2[:, 0] - u2i[:, 1]) + u2i[:, 1]
+    u2i[:, nyu + 1] = gi2 * (u2[:, nyv + 1] - u2i[:, nyu]) + u2i[:, nyu]
+    u3i = np.zeros((C, nyv + 2), complex)          # interp_u (FOU3D.f90:373-392)
+    for j in range(nyv + 2):
+        u3i[:, j] = ((yv[j] - yu[j]) * u3[:, j + 1] + (yu[j + 1] - yv[j]) * u3[:, j]) / (yu[j + 1] - yu[j])
+    out = {k: np.zeros(nyu + 2) for k in STATS_NAMES}
+    N = Mx * Mz
+    for j in range(nyu + 2):
+        if 1 <= j <= nyu:
+            out["Um"][j] = N * u1[0, j].real
+            out["U2m"][j] = Qp(u1[:, j], u1[:, j])
+            out["Wm"][j] = N * u3[0, j].real
+            out["W2m"][j] = Qp(u3[:, j], u3[:, j])
+            out["UVm"][j] = Qp(u1[:, j], u2i[:, j])
+            out["UWm"][j] = Qp(u1[:, j], u3[:, j])
+            out["Vm"][j] = N * u2[0, j].real
+            out["V2m"][j] = Qp(u2[:, j], u2[:, j])
+            out["VWm"][j] = Qp(u2[:, j], u3i[:, j])
+            out["Pm"][j] = ssum(p[:, j - 1])
+            out["P2m"][j] = sdot(p[:, j - 1], p[:, j - 1])
+        else:
+            out["Um"][j] = ssum(u1[:, j])
+            out["U2m"][j] = sdot(u1[:, j], u1[:, j])
+            out["Wm"][j] = ssum(u3[:, j])
+            out["W2m"][j] = sdot(u3[:, j], u3[:, j])
+            out["UVm"][j] = sdot(u1[:, j], u2i[:, j])
+            out["UWm"][j] = sdot(u1[:, j], u3[:, j])
+            if j == 0:
+                out["Vm"][j] = ssum(u2[:, 0])
+                out["V2m"][j] = sdot(u2[:, 0], u2[:, 0])
+                out["VWm"][j] = sdot(u2[:, 0], u3i[:, 0])
+        if j <= nyu:
+            w = -(1j * kz) * u2[:, j] + (u3[:, j + 1] - u3[:, j]) * dthdyv[j]
+            out["wxm"][j] = Nx * Nz * w[0].real
+            out["wx2m"][j] = Qn(w)
+    sp = {"spU": np.abs(u1) ** 2, "spV": np.abs(u2) ** 2, "spW": np.abs(u3) ** 2,
+          "spUV": u1.real * u2i.real + u1.imag * u2i.imag, "spP": np.abs(p) ** 2}  # spectra.f90:29-60
+    out.update(sp)
+    return out

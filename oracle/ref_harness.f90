@@ -18,7 +18,8 @@
 ! The time loop follows littleharsh.f90:163-233 without the hard-coded 4-step limit.
 !
 ! Input : params.txt (list-directed), state_in.bin (stream, complex(8) column fields, j fastest)
-! Output: state_out.bin, hist.txt, and for mode 2 stages.bin; for mode 3 fft_out.bin
+! Output: state_out.bin, hist.txt, and for mode 2 stages.bin; for mode 3 fft_out.bin; for mode 5
+! stats_out.bin
 program ref_harness
   use declaration
   implicit none
@@ -84,6 +85,43 @@ program ref_harness
   nstat = huge(nstat)
   nextqt = huge(nextqt)
   call divergence(div, u1, u2, u3, 0)
+
+  if (mode == 5) then
+    ! statistics and spectra of this state: the reference's own nonlinear at kRK = 1 with flagst = 1
+    ! (spectra FOU3D.f90:138-141, omega_x and stats FOU3D.f90:307-343); ppPL holds p in spectral planes
+    ! as after record_out's modes_to_planes_UVP (FOU3D.f90:151-152), the state of a write step.
+    allocate(Um(0:nyu+1), U2m(0:nyu+1), Wm(0:nyu+1), W2m(0:nyu+1), UVm(0:nyu+1), UWm(0:nyu+1))
+    allocate(Vm(0:nyu+1), V2m(0:nyu+1), VWm(0:nyu+1), wxm(0:nyu+1), wx2m(0:nyu+1))
+    allocate(Pm(1:nyu+1), P2m(1:nyu+1))
+    Um = 0d0; U2m = 0d0; Wm = 0d0; W2m = 0d0; UVm = 0d0; UWm = 0d0
+    Vm = 0d0; V2m = 0d0; VWm = 0d0; wxm = 0d0; wx2m = 0d0; Pm = 0d0; P2m = 0d0
+    allocate(spU(jlim(1,ugrid):jlim(2,ugrid),ncol), spV(jlim(1,vgrid):jlim(2,vgrid),ncol))
+    allocate(spW(jlim(1,ugrid):jlim(2,ugrid),ncol), spUV(jlim(1,ugrid):jlim(2,ugrid),ncol))
+    allocate(spP(jlim(1,pgrid):jlim(2,pgrid),ncol))
+    spU = 0d0; spV = 0d0; spW = 0d0; spUV = 0d0; spP = 0d0
+    allocate(wx(Nspec_x+2,Nspec_z,jgal(vgrid,1)-1:jgal(vgrid,2)+1))
+    allocate(u2PLN(Nspec_x+2,Nspec_z,jgal(vgrid,1)-1:jgal(vgrid,2)+1))
+    allocate(du3dy_planes(Nspec_x+2,Nspec_z,jgal(vgrid,1)-1:jgal(vgrid,2)+1))
+    allocate(du3dy_columns(jlim(1,vgrid):jlim(2,vgrid),ncol))
+    wx = 0d0; u2PLN = 0d0; du3dy_planes = 0d0; du3dy_columns = 0d0
+    istat = 0
+    nstat = 1
+    iter0 = iter - 1
+    ppPL = 0d0
+    call modes_to_planes_UVP(ppPL, p, 3, nyp, nyp_LB, 0, status, ierr)
+    kRK = 1
+    call RHS0_u1(du1, u1, Nu1, p, 0)
+    call RHS0_u2(du2, u2, Nu2, p, 0)
+    call RHS0_u3(du3, u3, Nu3, p, 0)
+    call nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, 0, status, ierr)
+    open(50, file='stats_out.bin', form='unformatted', access='stream', status='replace')
+    write(50) istat
+    write(50) Um, U2m, Wm, W2m, UVm, UWm, Vm, V2m, VWm, wxm, wx2m, Pm, P2m
+    write(50) spU, spV, spW, spUV, spP
+    close(50)
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
 
   if (mode == 4) then
     ! timing only: nsteps substeps (kRK = 1, 2, 3, 1, ...) of the main loop body, wall time of the loop

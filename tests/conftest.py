@@ -27,3 +27,22 @@ def golden():
     def load(name):
         return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
     return load
+
+
+def stats_errors(got, ref, npts_pad, npts_un):
+    """Largest error of each statistic relative to its scale; a mean against sqrt(points x largest
+    second moment), the size of its rounding in a physical-space sum."""
+    means = {"Um": ("U2m", npts_pad), "Wm": ("W2m", npts_pad), "Vm": ("V2m", npts_pad),
+             "wxm": ("wx2m", npts_un), "Pm": ("P2m", 1)}
+    err = {}
+    for name in ("Um", "U2m", "Wm", "W2m", "UVm", "UWm", "Vm", "V2m", "VWm", "wxm", "wx2m", "Pm", "P2m"):
+        r = np.asarray(ref[name])
+        g = np.asarray(got[name])
+        if len(g) == len(r) + 1:
+            g = g[1:]
+        scale = np.abs(r).max()
+        if name in means:
+            sq, npts = means[name]
+            scale = max(scale, np.sqrt(npts * np.abs(ref[sq]).max()))
+        err[name] = np.abs(g - r).max() / max(scale, 1e-300)
+    return err

@@ -12,6 +12,12 @@ Fixtures (inputs and expected outputs only):
   golden_grid_153.npz       : y grid stored in the reference's 32x32x153 fixture files.
   golden_fixture32_steps.npz: per-field norms and 8 columns of the reference run of 4 steps from its
                               own 32x32x153 fixture (input/*_0032x0032x0153_t00000.dat).
+  golden_stats_24x24x17.npz : a state (the reference's fields 3 steps after the seeded IC, seed 21) and
+                              the reference's statistics and spectra of it (stats.f90, spectra.f90,
+                              omega_x of FOU3D.f90:307-338) from one kRK = 1 nonlinear call with the
+                              statistics flag set, p in its planes as on a write step.
+
+``python make_golden.py stats`` writes only the last one.
 """
 import os
 import sys
@@ -30,9 +36,26 @@ import pyoracle  # noqa: E402
 REF_INPUT = "/root/reference/input"
 
 
+def stats_golden():
+    Mx = Mz = 24
+    ny = 17
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=21, amp=0.1, broadband=True)
+    r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=3)
+    state = [r[k] for k in ("u1", "u2", "u3", "p")]
+    s = pyoracle.run_reference(Mx, Mz, ny, *state, mode=5, t=r["t"], it=3)
+    out = {"u1_in": state[0], "u2_in": state[1], "u3_in": state[2], "p_in": state[3], "istat": s["istat"]}
+    for k in pyoracle.STATS_NAMES + ("spU", "spV", "spW", "spUV", "spP"):
+        out[k] = s[k]
+    np.savez_compressed(os.path.join(HERE, "golden_stats_24x24x17.npz"), **out)
+
+
 def main():
     if not pyoracle.have_reference():
         raise SystemExit("oracle/_ref/ref_harness is not built")
+    if sys.argv[1:] == ["stats"]:
+        stats_golden()
+        return
+    stats_golden()
     Mx = Mz = 24
     ny = 17
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=7, amp=0.3, nmodes=4)

@@ -12,7 +12,7 @@ import dns_ic
 import dns_io
 import dnship
 import pyoracle
-from conftest import PKG, rel_l2
+from conftest import PKG, rel_l2, stats_errors
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(PKG, "fortran", "littleharsh_hip")
@@ -76,6 +76,46 @@ def test_fortran_driver_matches_oracle(tmp_path, mode):
     assert rows[0]["t"] == 0.0 and abs(rows[-1]["t"] - o.scalars()["t"]) <= 1e-13
     assert abs(rows[-1]["Qx"] - o.scalars()["Qx"]) <= 1e-13 * abs(o.scalars()["Qx"])
     assert all(r["flag_ctpress"] == 1 for r in rows) and hh["Re"] == 3250.0
+
+
+def test_fortran_driver_stats_files_vs_oracle(tmp_path):
+    """Statistics and spectra files of the driver (every nstat = min(20, nwrite) = 1 steps, written and
+    reset at each write step) against the oracle's statistics of the state at the file's iteration."""
+    Mx, Mz, ny, n = 48, 48, 33, 4
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    st, tag = write_case(str(tmp_path), Mx, Mz, ny, seed=13)
+    print("case: fortran driver statistics files %dx%dx%d %d steps" % (Mx, Mz, ny, n), flush=True)
+    r = subprocess.run([DRIVER, str(n), "fused"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    outdir = os.path.join(str(tmp_path), "output")
+    files = sorted(f for f in os.listdir(outdir) if f.startswith("stats_"))
+    assert files
+    o = pyoracle.Oracle(Mx, Mz, ny)
+    grid = o.setup()
+    o.set_state(*st)
+    done = 0
+    seen = set()
+    for f in sorted(files, key=lambda f: dns_io.read_stats(os.path.join(outdir, f))["iter"]):
+        s = dns_io.read_stats(os.path.join(outdir, f))
+        assert s["istat"] == (1 if s["iter"] >= 1 else 0)
+        if s["istat"] == 0:
+            assert all(not np.any(s[k]) for k in dns_io.STATS_FILE_ORDER)
+            continue
+        o.step(s["iter"] - done)
+        done = s["iter"]
+        seen.add(done)
+        ref = pyoracle.plane_stats(Mx, Mz, ny, *o.get_state(), grid["yu"], grid["yv"], grid["dthdyv"], 2.0)
+        nyu = ny - 1
+        exp = {k: ref[k][:nyu + 1] if k in ("Vm", "V2m", "VWm", "wxm", "wx2m") else
+               ref[k][1:nyu + 1] if k in ("Pm", "P2m") else ref[k] for k in dns_io.STATS_FILE_ORDER}
+        err = stats_errors(s, exp, Mx * Mz, Nx * Nz)
+        assert max(err.values()) <= 1e-11, (f, err)
+        for name in ("spU", "spV", "spW", "spUV", "spP"):
+            sp = dns_io.read_spectrum(os.path.join(outdir, name + f[5:]), Nx, Nz)
+            want = dns_io.fold_spectrum(ref[name], Nx, Nz)
+            assert sp["iter"] == done and sp["istat"] == 1
+            assert np.abs(sp["sp"] - want).max() <= 1e-11 * np.abs(want).max(), name
+    assert seen, files
 
 
 def test_fortran_driver_rejects_bad_size(tmp_path):

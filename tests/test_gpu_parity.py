@@ -10,7 +10,7 @@ import pytest
 import dns_ic
 import dnship
 import pyoracle
-from conftest import rel_l2
+from conftest import rel_l2, stats_errors
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -398,6 +398,80 @@ def test_history_record_linear_profile():
     assert abs(h["Uslip"] - (a - b)) <= 1e-14
     assert h["Umax"] == (a + b * yu).max()
     assert h["iter"] == 0 and h["t"] == 0.0
+
+
+def _stats_close(got, ref, npts_pad, npts_un, tol=1e-12):
+    err = stats_errors(got, ref, npts_pad, npts_un)
+    assert max(err.values()) <= tol, err
+
+
+def test_stats_and_spectra_vs_reference_golden(golden):
+    """stats.f90 / spectra.f90 / omega_x of the reference, run by the reference's own code on a state
+    3 steps after a seeded IC (golden_stats_24x24x17.npz)."""
+    gd = golden("golden_stats_24x24x17.npz")
+    Mx = Mz = 24
+    ny = 17
+    print("case: statistics 24x24x17", flush=True)
+    g = dnship.DNS(Mx, Mz, ny)
+    g.upload(gd["u1_in"], gd["u2_in"], gd["u3_in"], gd["p_in"])
+    g.accumulate_stats()
+    s = g.stats(spectra=True)
+    assert s["istat"] == int(gd["istat"]) == 1
+    _stats_close(s, gd, Mx * Mz, (2 * Mx // 3) * (2 * Mz // 3))
+    for k in ("spU", "spV", "spW", "spUV", "spP"):
+        assert np.abs(s[k] - gd[k]).max() <= 1e-13 * np.abs(gd[k]).max(), k
+    g.accumulate_stats()               # accumulation: the same state twice doubles every sum
+    s2 = g.stats(spectra=True)
+    assert s2["istat"] == 2
+    for k in dnship.DNS.STATS + ("spU", "spP"):
+        assert np.allclose(s2[k], 2 * s[k], rtol=1e-15, atol=0), k
+    g.reset_stats()
+    s3 = g.stats(spectra=True)
+    assert s3["istat"] == 0 and all(not np.any(s3[k]) for k in dnship.DNS.STATS + ("spU",))
+    g.close()
+
+
+def test_stats_after_steps_vs_oracle():
+    """Statistics and spectra of a broadband state after two steps, against the oracle's restatement of
+    stats.f90 / spectra.f90 on the downloaded state."""
+    Mx, Mz, ny = 96, 96, 65
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=41, broadband=True)
+    print("case: statistics after steps 96x96x65", flush=True)
+    g = dnship.DNS(Mx, Mz, ny)
+    g.upload(*st)
+    g.step(2)
+    g.accumulate_stats()
+    s = g.stats(spectra=True)
+    cur = g.download()
+    g.close()
+    o = pyoracle.Oracle(Mx, Mz, ny).setup()
+    ref = pyoracle.plane_stats(Mx, Mz, ny, *cur, o["yu"], o["yv"], o["dthdyv"], 2.0)
+    ref["Pm"], ref["P2m"] = ref["Pm"][1:], ref["P2m"][1:]
+    _stats_close(s, ref, Mx * Mz, (2 * Mx // 3) * (2 * Mz // 3))
+    for k in ("spU", "spV", "spW", "spUV", "spP"):
+        assert np.abs(s[k] - ref[k]).max() <= 1e-14 * np.abs(ref[k]).max(), k
+
+
+def test_stats_ensemble_members_independent():
+    """Each member accumulates its own statistics; equal to the same channel alone, bit for bit."""
+    Mx, Mz, ny = 48, 48, 33
+    sts = [dns_ic.seeded_channel(Mx, Mz, ny, seed=s, broadband=True) for s in (31, 32)]
+    print("case: statistics ensemble", flush=True)
+    e = dnship.DNS(Mx, Mz, ny, members=2)
+    for m, st in enumerate(sts):
+        e.upload(*st, member=m)
+    e.step(2)
+    e.accumulate_stats()
+    for m, st in enumerate(sts):
+        g = dnship.DNS(Mx, Mz, ny)
+        g.upload(*st)
+        g.step(2)
+        g.accumulate_stats()
+        a, b = e.stats(member=m, spectra=True), g.stats(spectra=True)
+        g.close()
+        for k in dnship.DNS.STATS + ("spU", "spV", "spW", "spUV", "spP"):
+            assert np.array_equal(a[k], b[k]), (m, k)
+    e.close()
 
 
 def test_restart_bit_exact():

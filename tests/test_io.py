@@ -72,3 +72,22 @@ def test_field_file_round_trip(tmp_path):
     g = dns_io.read_field(path)
     assert (g.t, g.Re, g.iter, g.nband) == (1.25, 3250.0, 7, 3)
     assert np.array_equal(g.planes, planes) and np.array_equal(g.N, N) and np.array_equal(g.y, y)
+
+
+def test_fold_spectrum_known_answer():
+    """recvwrspec's folding (spectra.f90:142-165): +kz and -kz share a slot, i >= 1 and kz < Nz/2
+    doubled, the kz = -Nz/2 row and i = 0 kept single."""
+    Nx, Nz = 8, 6
+    Kx = Nx // 2 + 1
+    sp = np.zeros((Kx * Nz, 2))
+    col = lambda i, kz: (kz if kz >= 0 else kz + Nz) * Kx + i
+    sp[col(1, 1), 0] = 1.0
+    sp[col(1, -1), 0] = 10.0
+    sp[col(0, 0), 1] = 3.0
+    sp[col(2, -3), 1] = 5.0           # kz = -Nz/2
+    sp[col(0, 2), 1] = 7.0
+    b = dns_io.fold_spectrum(sp, Nx, Nz)
+    assert b.shape == (2, Nz // 2 + 1, Kx)
+    assert b[0, 1, 1] == 22.0 and np.count_nonzero(b[0]) == 1
+    assert b[1, 0, 0] == 3.0 and b[1, Nz // 2, 2] == 5.0 and b[1, 2, 0] == 7.0
+    assert np.count_nonzero(b[1]) == 3

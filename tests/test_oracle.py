@@ -14,7 +14,7 @@ import pytest
 
 import dns_ic
 import pyoracle
-from conftest import rel_l2
+from conftest import rel_l2, stats_errors
 
 TOL = 1e-12
 
@@ -182,6 +182,43 @@ def test_oracle_projection_is_divergence_free():
     d = dns_ic.divergence(u1, u2, u3, o.Nx, o.Nz, ny)
     assert np.abs(d).max() <= 1e-10
     assert np.abs(u2[:, 0]).max() == 0.0 and np.abs(u2[:, -1]).max() == 0.0
+
+
+def _plane_stats(Mx, Mz, ny, st, bet=2.0):
+    s = pyoracle.Oracle(Mx, Mz, ny).setup()
+    return pyoracle.plane_stats(Mx, Mz, ny, *st, s["yu"], s["yv"], s["dthdyv"], bet)
+
+
+def test_oracle_plane_stats_vs_reference_golden(golden):
+    """The sum-rule statistics and the spectra against stats.f90 / spectra.f90 run by the reference's
+    own code (physical-space sums after its FFTs) on a state 3 steps after a seeded IC."""
+    g = golden("golden_stats_24x24x17.npz")
+    o = _plane_stats(24, 24, 17, (g["u1_in"], g["u2_in"], g["u3_in"], g["p_in"]))
+    err = stats_errors(o, g, 24 * 24, 16 * 16)
+    assert max(err.values()) < 1e-12, err
+    for k in ("spU", "spV", "spW", "spUV", "spP"):
+        assert np.abs(o[k] - g[k]).max() <= 1e-14 * np.abs(g[k]).max(), k
+
+
+def test_plane_stats_planted_mistake_is_caught(golden):
+    """A wrong spanwise wavenumber in omega_x must break the golden comparison of wxm/wx2m."""
+    g = golden("golden_stats_24x24x17.npz")
+    o = _plane_stats(24, 24, 17, (g["u1_in"], g["u2_in"], g["u3_in"], g["p_in"]), bet=2.0 * (1 + 1e-6))
+    err = stats_errors(o, g, 24 * 24, 16 * 16)
+    assert err["wx2m"] > 1e-9 and err["U2m"] < 1e-12
+
+
+def test_plane_stats_mean_flow_closed_form():
+    """A pure mean flow u1(y) on the interior planes: Um = Mx Mz U(y), U2m = Mx Mz U(y)^2, nothing else."""
+    Mx, Mz, ny = 24, 24, 17
+    st = [np.zeros_like(a) for a in seeded(Mx, Mz, ny, 3)]
+    U = np.linspace(0.3, 1.1, ny + 1)
+    st[0][0, :] = U
+    o = _plane_stats(Mx, Mz, ny, st)
+    j = np.arange(1, ny)
+    assert np.allclose(o["Um"][j], Mx * Mz * U[j], rtol=1e-15)
+    assert np.allclose(o["U2m"][j], Mx * Mz * U[j] ** 2, rtol=1e-14)
+    assert np.all(o["UVm"] == 0) and np.all(o["W2m"] == 0) and np.all(o["wx2m"] == 0)
 
 
 def test_planted_mistake_is_caught(golden):
