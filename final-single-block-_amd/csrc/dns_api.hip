@@ -997,6 +997,69 @@ extern "C" int dns_get_stats(dns_handle *h, int m, double *stats, double *spectr
   return DNS_OK;
 }
 
+// ---- four_to_phys post-processing (four_to_phys.f90, four_to_phys_N.f90) --------------------------
+extern "C" int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, const double *in, double *out,
+                                double *ms) {
+  if (nplanes < 0 || (nplanes > 0 && (!in || !out))) return fail(DNS_ERR_ARG, "dns_four_to_phys: bad argument");
+  if (Nx < 2 || Nz < 2 || Nx % 2 || Nz % 2 || Mx < Nx || Mz < Nz)
+    return fail(DNS_ERR_ARG, "dns_four_to_phys: sizes %d x %d -> %d x %d (need even N <= M)", Nx, Nz, Mx, Mz);
+  if (!dnsk::four_to_phys_supported(Mx, Mz))
+    return fail(DNS_ERR_ARG, "dns_four_to_phys: %d x %d is not a supported transform size (Mz and Mx/2 in "
+                "8..1152, 2^a 3^b)", Mx, Mz);
+  if (ms) *ms = 0.0;
+  if (nplanes == 0) return DNS_OK;
+  const size_t KX = Nx / 2 + 1, nin = (size_t)(Nx + 2) * Nz, nout = (size_t)(Mx + 2) * Mz, nmid = KX * Mz;
+  // planes per batch: at most ~1 GiB of device buffers
+  const size_t per = (nin + nout) * sizeof(double) + nmid * sizeof(double2);
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nplanes, ((size_t)1 << 30) / per));
+  double *din = nullptr, *dout = nullptr;
+  double2 *dmid = nullptr, *tw = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DNS_OK;
+  auto hip_ok = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && rc == DNS_OK) rc = fail(DNS_ERR_HIP, "dns_four_to_phys: %s: %s", what, hipGetErrorString(e));
+    return rc == DNS_OK;
+  };
+  const auto a = twiddles<double>(Mz), b = twiddles<double>(Mx), c = twiddles<double>(Mx / 2);
+  if (hip_ok(hipMalloc(&din, nb * nin * sizeof(double)), "hipMalloc") &&
+      hip_ok(hipMalloc(&dout, nb * nout * sizeof(double)), "hipMalloc") &&
+      hip_ok(hipMalloc(&dmid, nb * nmid * sizeof(double2)), "hipMalloc") &&
+      hip_ok(hipMalloc(&tw, (size_t)(Mz + Mx + Mx / 2) * sizeof(double2)), "hipMalloc") &&
+      hip_ok(hipMemcpy(tw, a.data(), Mz * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy") &&
+      hip_ok(hipMemcpy(tw + Mz, b.data(), Mx * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy") &&
+      hip_ok(hipMemcpy(tw + Mz + Mx, c.data(), (Mx / 2) * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy") &&
+      hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate")) {
+    double total = 0.0;
+    for (int p0 = 0; p0 < nplanes && rc == DNS_OK; p0 += nb) {
+      const int n = std::min(nb, nplanes - p0);
+      if (!hip_ok(hipMemcpy(din, in + (size_t)p0 * nin, n * nin * sizeof(double), hipMemcpyHostToDevice), "upload"))
+        break;
+      dnsk::FtpArgs fa{Nx, Nz, Mx, Mz, n, din, dmid, dout, tw, tw + Mz, tw + Mz + Mx};
+      (void)hipEventRecord(e0, 0);
+      if (dnsk::launch_four_to_phys(fa, 0)) {
+        rc = fail(DNS_ERR_ARG, "dns_four_to_phys: %d x %d not instantiated", Mx, Mz);
+        break;
+      }
+      (void)hipEventRecord(e1, 0);
+      if (!hip_ok(hipGetLastError(), "launch") ||
+          !hip_ok(hipMemcpy(out + (size_t)p0 * nout, dout, n * nout * sizeof(double), hipMemcpyDeviceToHost),
+                  "download"))
+        break;
+      float t = 0.0f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      total += t;
+    }
+    if (ms && rc == DNS_OK) *ms = total;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  (void)hipFree(dmid);
+  (void)hipFree(tw);
+  return rc;
+}
+
 extern "C" int dns_set_column_path(dns_handle *h, int path) {
   if (!h) return fail(DNS_ERR_ARG, "dns_set_column_path: null handle");
   if (path != 0 && path != 1) return fail(DNS_ERR_ARG, "dns_set_column_path: path=%d must be 0 or 1", path);

@@ -188,6 +188,17 @@ struct StatsArgs {
 int stats_nchunk(int C);
 void launch_stats(const StatsArgs &a, hipStream_t st);
 
+// four_to_phys post-processing (post_kernels.hip): n spectral planes (Nx+2) x Nz -> physical (Mx+2) x Mz
+struct FtpArgs {
+  int Nx, Nz, Mx, Mz, n;
+  const double *in;
+  double2 *mid;      // z-inverted columns [plane][Mz][Nx/2+1]
+  double *out;
+  const double2 *twz, *twx, *twh;  // e^{-2 pi i m/N} for N = Mz, Mx, Mx/2
+};
+bool four_to_phys_supported(int Mx, int Mz);
+int launch_four_to_phys(const FtpArgs &a, hipStream_t st);
+
 // one-pass column stage (column_pass.hip): flag_ctpress = 1, 8 <= nyu <= 512
 bool column_pass_supported(int nyu);
 int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered

@@ -84,8 +84,9 @@ def read_field(path: str) -> FieldFile:
                      np.array(js), np.array(planes), np.array(ys))
 
 
-def write_field(path: str, ff: FieldFile, dummint: np.ndarray | None = None) -> None:
-    """Write a field file with the record layout of record_out (FOU3D.f90:2063-2161)."""
+def write_field(path: str, ff: FieldFile, dummint: np.ndarray | None = None, grid_id: int = 1) -> None:
+    """Write a field file with the record layout of record_out (FOU3D.f90:2063-2161); grid_id is the
+    second integer of each plane record (record_out writes 1, 2, 3, 4 for u1, u2, u3, p)."""
     def rec(payload: bytes) -> bytes:
         return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
 
@@ -96,7 +97,6 @@ def write_field(path: str, ff: FieldFile, dummint: np.ndarray | None = None) -> 
     out.append(rec(np.asarray(ff.N, "<i4").T.tobytes()))
     out.append(rec(np.asarray(ff.y, "<f8").tobytes() + struct.pack("<d", ff.ddthetavi)
                    + np.asarray(ff.dthdy, "<f8").tobytes()))
-    grid_id = 1
     for jj, yj, pl in zip(ff.j, ff.plane_y, ff.planes):
         nz, kx = pl.shape
         b = np.empty((nz, 2 * kx), dtype="<f8")

@@ -45,7 +45,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_set_timing", "dns_get_timing", "dns_get_plane_variants", "dns_set_plane_variants",
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
            "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
-           "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats"]
+           "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats",
+           "dns_four_to_phys"]
 
 
 def lib():
@@ -93,6 +94,7 @@ def lib():
     L.dns_accumulate_stats.argtypes = [vp]
     L.dns_reset_stats.argtypes = [vp]
     L.dns_get_stats.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.dns_four_to_phys.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -113,6 +115,18 @@ def ygrid(ny, dyq=10.0, ppp=5):
     _chk(lib().dns_ygrid(int(ny), float(dyq), int(ppp), _p(yu), _p(yv), _p(du), _p(dv)))
     return dict(yu=yu, yv=yv, dthdyu=du, dthdyv=dv)
 
+
+
+def four_to_phys(planes, Mx, Mz):
+    """Spectral planes of a field file [n, Nz, Nx+2] (float64, the file's buffSR1 layout) to physical planes
+    [n, Mz, Mx+2] on the GPU (four_to_phys.f90 with Mx, Mz = 3/2 Nx, Nz; four_to_phys_N.f90 with Mx, Mz =
+    Nx, Nz).  Returns (planes, device milliseconds of the transform kernels)."""
+    planes = np.ascontiguousarray(planes, dtype=np.float64)
+    n, Nz, nx = planes.shape
+    out = np.empty((n, Mz, Mx + 2))
+    ms = ctypes.c_double()
+    _chk(lib().dns_four_to_phys(nx - 2, Nz, Mx, Mz, n, _p(planes), _p(out), ctypes.byref(ms)))
+    return out, ms.value
 
 class DNS:
     """One channel on the GPU (one dns_handle)."""

@@ -186,6 +186,15 @@ module dns_hip
       type(c_ptr), value :: stats, spectra
       integer(c_int), intent(out) :: istat
     end function
+    ! fin(Nx+2, Nz, nplanes) spectral planes -> fout(Mx+2, Mz, nplanes) physical planes
+    integer(c_int) function dns_four_to_phys(Nx, Nz, Mx, Mz, nplanes, fin, fout, ms) &
+        bind(C, name='dns_four_to_phys')
+      import :: c_int, c_double
+      integer(c_int), value :: Nx, Nz, Mx, Mz, nplanes
+      real(c_double), intent(in) :: fin(*)
+      real(c_double), intent(out) :: fout(*)
+      real(c_double), intent(out) :: ms
+    end function
     integer(c_int) function dns_set_column_path(h, path) bind(C, name='dns_set_column_path')
       import :: c_int, c_ptr
       type(c_ptr), value :: h

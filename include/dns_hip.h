@@ -159,6 +159,15 @@ int dns_accumulate_stats(dns_handle *h);
 int dns_get_stats(dns_handle *h, int m, double *stats, double *spectra, int *istat);
 int dns_reset_stats(dns_handle *h);
 
+/* four_to_phys post-processing: the per-plane resize + cft + rft of the reference's post-processors
+ * (four_to_phys.f90:60-81 and four_to_phys_N.f90:60-81, resize at four_to_phys.f90:318-337), for a batch of
+ * planes on the GPU; needs no handle.  in: nplanes spectral planes buffSR1(Nx+2, Nz) as in a field file
+ * (host memory); out: nplanes physical planes buffSR2(Mx+2, Mz).  Mx >= Nx, Mz >= Nz (Mz and Mx/2 in
+ * 8..1152, of the form 2^a 3^b); the kz = -Nz/2 row is dropped when Mz > Nz (four_to_phys, Ngal = 3N/2)
+ * and kept when Mz == Nz (four_to_phys_N, Ngal = N).  ms (may be NULL): device time of the transform
+ * kernels in ms. */
+int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, const double *in, double *out, double *ms);
+
 /* Column stage of the fused path: path 0 = one kernel per substep (rows split into segments, tridiagonal
  * systems by the partition method, everything read and written once), 1 = three sweeps (thread per
  * column, reference elimination order).  Same equations as RHS0 + nonlinear (column part) + solveU +

@@ -379,3 +379,119 @@ def plane_stats(Mx, Mz, ny, u1, u2, u3, p, yu, yv, dthdyv, bet):
           "spUV": u1.real * u2i.real + u1.imag * u2i.imag, "spP": np.abs(p) ** 2}  # spectra.f90:29-60
     out.update(sp)
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+def _frecords(buf):
+    import struct
+    off, out = 0, []
+    while off < len(buf):
+        (n,) = struct.unpack_from("<i", buf, off)
+        out.append(buf[off + 4:off + 4 + n])
+        off += n + 8
+    return out
+
+
+def _frec(payload):
+    import struct
+    return struct.pack("<i", len(payload)) + payload + struct.pack("<i", len(payload))
+
+
+def ftp_plane(b, Mx, Mz):
+    """resize + cft + rft of four_to_phys.f90:76-79 (resize at :318-337, four_to_phys_N.f90's resize keeps
+    the kz = -Nz/2 row, which is what Mz == Nz selects here) on one spectral plane b[z, x] (Nz x (Nx+2)):
+    unnormalised inverse transforms, z then x; the real x-inverse takes Re of kx = 0 and of kx = Mx/2,
+    and entries Mx, Mx+1 keep the kx = Mx/2 coefficient.  Returns the physical plane [Mz, Mx+2]."""
+    Nz, nx = b.shape
+    Nx = nx - 2
+    KX = Nx // 2 + 1
+    H = Mx // 2
+    c = b[:, 0::2] + 1j * b[:, 1::2]
+    full = np.zeros((Mz, KX), complex)
+    full[:Nz // 2] = c[:Nz // 2]
+    lo = Mz - Nz // 2 + (1 if Mz > Nz else 0)
+    full[lo:] = c[lo + Nz - Mz:]
+    zi = Mz * np.fft.ifft(full, axis=0)
+    cx = np.zeros((Mz, H + 1), complex)
+    cx[:, :KX] = zi
+    out = np.zeros((Mz, Mx + 2))
+    out[:, :Mx] = Mx * np.fft.irfft(cx, n=Mx, axis=1)
+    out[:, Mx] = cx[:, H].real
+    out[:, Mx + 1] = cx[:, H].imag
+    return out
+
+
+def four_to_phys_files(files, nvar=False):
+    """four_to_phys.f90 (nvar: four_to_phys_N.f90) on the four field files {u1, u2, u3, p: bytes}: the
+    output files' bytes, record for record (headers of u2, u3 and p from u1's; planes in the order of the
+    band loops, :60-113 for u1, :116-186 for u2, :188-257 for u3, :260-305 for p)."""
+    import struct
+    out = {}
+    r1 = _frecords(files["u1"])
+    head = r1[0]
+    nband = struct.unpack_from("<i", head, 40)[0]
+    N = np.frombuffer(r1[1], "<i4").reshape(nband + 2, 4).T
+    Ngal = N.copy()
+    if not nvar:
+        Ngal[0, 1:nband + 1] = N[0, 1:nband + 1] * 3 // 2
+        Ngal[1, 1:nband + 1] = N[1, 1:nband + 1] * 3 // 2
+    Nx, Nz = int(N[0, 1]), int(N[1, 1])
+    Mx, Mz = int(Ngal[0, 1]), int(Ngal[1, 1])
+    ylen_u = int(N[3, nband]) + 2 - int(N[3, 0])
+    ryu = r1[2]
+    yu = np.frombuffer(ryu, "<f8")[:ylen_u]
+    dthetai_u1 = ryu[8 * ylen_u:8 * ylen_u + 8]
+    r2 = _frecords(files["u2"])
+    ylen_v = int(N[2, nband]) + 2 - int(N[2, 0])
+    yv = np.frombuffer(r2[2], "<f8")[:ylen_v]
+    dthetai_u2 = r2[2][8 * ylen_v:8 * ylen_v + 8]
+
+    def planes(recs):
+        for r in recs[3:]:
+            jd, gid, nx, nz = struct.unpack_from("<4i", r, 0)
+            (y,) = struct.unpack_from("<d", r, 16)
+            yield jd, gid, y, np.frombuffer(r, "<f8", offset=24, count=(Nx + 2) * Nz).reshape(Nz, Nx + 2)
+
+    def plane_rec(jd, gid, y, b):
+        return _frec(struct.pack("<4id", jd, gid, Mx + 2, Mz, y) + ftp_plane(b, Mx, Mz).astype("<f8").tobytes())
+
+    # u3 and p write u1's y record with the dthetai last read (u2's): the programs share one variable
+    yrec_u3p = ryu[:8 * ylen_u] + dthetai_u2 + ryu[8 * ylen_u + 8:]
+    for name, recs, yrec in (("u1", r1, ryu), ("u2", r2, r2[2]), ("u3", _frecords(files["u3"]), yrec_u3p)):
+        it = planes(recs)
+        row = 2 if name == "u2" else 3
+        y = yv if name == "u2" else yu
+        y0 = int(N[row, 0])
+        o = [_frec(head), _frec(Ngal.T.astype("<i4").tobytes()), _frec(yrec)]
+        last = None
+        for iband in range(1, nband + 1):
+            if iband == 1:
+                last = next(it)
+                o.append(plane_rec(*last))
+            else:
+                jj = int(N[row, iband - 1])
+                if name == "u2":
+                    o.append(plane_rec(last[0], last[1], y[jj - y0], last[3]))
+                else:
+                    o.append(plane_rec(jj, 1, y[jj - y0], last[3]))
+            for _ in range(int(N[row, iband - 1]) + 1, int(N[row, iband]) + 1):
+                last = next(it)
+                o.append(plane_rec(*last))
+            if iband == nband:
+                last = next(it)
+                o.append(plane_rec(*last))
+        out[name] = b"".join(o)
+    it = planes(_frecords(files["p"]))
+    o = [_frec(head), _frec(Ngal.T.astype("<i4").tobytes()), _frec(yrec_u3p)]
+    N4 = [int(v) for v in N[3]]
+    for iband in range(1, nband + 1):
+        if iband == 1:
+            rng = range(N4[0] + 1, N4[1])
+        elif iband == 3:
+            rng = range(N4[2] + 2, N4[3] + 1) if nvar else range(N4[2] + 1, N4[3])
+        else:
+            rng = range(N4[1], N4[2] + 2) if nvar else range(N4[1], N4[2] + 1)
+        for _ in rng:
+            o.append(plane_rec(*next(it)))
+    out["p"] = b"".join(o)
+    return out

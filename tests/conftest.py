@@ -46,3 +46,31 @@ def stats_errors(got, ref, npts_pad, npts_un):
             scale = max(scale, np.sqrt(npts * np.abs(ref[sq]).max()))
         err[name] = np.abs(g - r).max() / max(scale, 1e-300)
     return err
+
+
+def ftp_compare(got: bytes, ref: bytes, tol: float):
+    """Two four_to_phys output files record by record: the header records and every plane record's
+    (jd, id, nx, nz, y) byte for byte, the plane values within tol of the plane's largest value."""
+    import struct
+
+    def recs(buf):
+        off, out = 0, []
+        while off < len(buf):
+            (n,) = struct.unpack_from("<i", buf, off)
+            out.append(buf[off + 4:off + 4 + n])
+            off += n + 8
+        return out
+    a, b = recs(got), recs(ref)
+    assert len(a) == len(b), (len(a), len(b))
+    worst = 0.0
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert len(x) == len(y), k
+        if k < 3:
+            assert x == y, "header record %d" % k
+            continue
+        assert x[:24] == y[:24], (k, struct.unpack_from("<4id", x), struct.unpack_from("<4id", y))
+        u, v = np.frombuffer(x, "<f8", offset=24), np.frombuffer(y, "<f8", offset=24)
+        worst = max(worst, float(np.abs(u - v).max() / max(np.abs(v).max(), 1e-300)))
+    assert worst <= tol, worst
+    return worst
+

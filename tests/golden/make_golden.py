@@ -17,8 +17,16 @@ Fixtures (inputs and expected outputs only):
                               omega_x of FOU3D.f90:307-338) from one kRK = 1 nonlinear call with the
                               statistics flag set, p in its planes as on a write step.
 
-``python make_golden.py stats`` writes only the last one.
+  golden_ftp_16x16x9.npz    : the reference's post-processors (four_to_phys, four_to_phys_N, compiled
+                              from their sources into oracle/_ref) on small random field files: the
+                              input and output files, byte for byte.  Case A: four_to_phys on files
+                              with record_out's current header; case B: four_to_phys_N on files with
+                              a three-band header like the reference's own input files.
+
+``python make_golden.py stats`` (or ``ftp``) writes only that one.
 """
+import subprocess
+import tempfile
 import os
 import sys
 
@@ -49,13 +57,87 @@ def stats_golden():
     np.savez_compressed(os.path.join(HERE, "golden_stats_24x24x17.npz"), **out)
 
 
+FTP_NX, FTP_NZ, FTP_NY = 16, 16, 9
+FTP_FIELDS = (("u1", 1), ("u2", 2), ("u3", 3), ("p", 4))
+
+
+def ftp_case_files(case, seed):
+    """Random field files (every entry set, including those the transforms ignore) for one case."""
+    Nx, Nz, ny = FTP_NX, FTP_NZ, FTP_NY
+    nyu, nyv = ny - 1, ny - 2
+    N = np.zeros((4, 5), np.int32)
+    if case == "A":      # record_out's header (FOU3D.f90:2024-2030)
+        N[0, 1:4] = Nx
+        N[0, 4] = -2
+        N[1, 1:4] = Nz
+        N[2, 3] = nyv
+        N[3, 3] = nyu
+    else:                # three bands, as in the reference's input/*.dat files
+        N[:, 1] = [Nx, Nz, 2, 3]
+        N[:, 2] = [Nx, Nz, 5, 5]
+        N[:, 3] = [Nx, Nz, nyv, nyu]
+        N[:, 4] = [-2, 0, 0, 0]
+    rng = np.random.default_rng(seed)
+    yu = np.sort(rng.uniform(-1, 1, nyu + 2))
+    yv = np.sort(rng.uniform(-1, 1, nyv + 2))
+    files = {}
+    tag = "_%04dx%04dx%04d_t00000.dat" % (Nx, Nz, ny)
+    for name, gid in FTP_FIELDS:
+        y, js = (yv, np.arange(0, nyv + 2)) if name == "u2" else (yu, np.arange(1, nyu + 1) if name == "p"
+                                                                   else np.arange(0, nyu + 2))
+        planes = rng.standard_normal((len(js), Nz, Nx // 2 + 1)) + 1j * rng.standard_normal((len(js), Nz, Nx // 2 + 1))
+        ff = dns_io.FieldFile(0.004, 3250.0, 1.0, 2.0, -3.0675e-3, 3, 17, N, y, 1.0, 0.5 + 0 * y, js, planes, y[js])
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "f.dat")
+            dns_io.write_field(path, ff, grid_id=gid)
+            with open(path, "rb") as f:
+                files[name + tag] = f.read()
+    return files
+
+
+def ftp_input_text():
+    return "mgalx1=%10d\nmgalz1=%10d\nny=    %10d\nt=     %10s\n" % (FTP_NX, FTP_NZ, FTP_NY, "0.0d0")
+
+
+def ftp_golden():
+    out = {}
+    for case, prog, seed in (("A", "four_to_phys", 3), ("B", "four_to_phys_N", 4)):
+        files = ftp_case_files(case, seed)
+        with tempfile.TemporaryDirectory() as d:
+            os.makedirs(os.path.join(d, "output"))
+            # The programs re-OPEN unit 10 on the next field's file without a CLOSE.  amdflang's runtime
+            # then keeps reading at the previous file's end offset instead of the new file's start, so
+            # each input file is preceded by as many zero bytes as the files read before it (the
+            # programs write unit 20 from the start of each new file, so the outputs need nothing).
+            pad = 0
+            for name, _ in FTP_FIELDS:
+                fn = [k for k in files if k.startswith(name + "_")][0]
+                with open(os.path.join(d, "output", fn), "wb") as f:
+                    f.write(b"\0" * pad + files[fn])
+                pad += len(files[fn])
+            with open(os.path.join(d, "input_FtP.txt"), "w") as f:
+                f.write(ftp_input_text())
+            subprocess.run([os.path.join(ROOT, "oracle", "_ref", prog)], cwd=d, check=True,
+                           capture_output=True, timeout=120)
+            for fn, b in files.items():
+                name = fn.split("_")[0]
+                out["%s_in_%s" % (case, name)] = np.frombuffer(b, np.uint8)
+                with open(os.path.join(d, "output", fn.replace(name + "_", name + "_phys_", 1)), "rb") as f:
+                    out["%s_out_%s" % (case, name)] = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "golden_ftp_16x16x9.npz"), **out)
+
+
 def main():
     if not pyoracle.have_reference():
         raise SystemExit("oracle/_ref/ref_harness is not built")
     if sys.argv[1:] == ["stats"]:
         stats_golden()
         return
+    if sys.argv[1:] == ["ftp"]:
+        ftp_golden()
+        return
     stats_golden()
+    ftp_golden()
     Mx = Mz = 24
     ny = 17
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=7, amp=0.3, nmodes=4)

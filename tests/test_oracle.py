@@ -14,7 +14,7 @@ import pytest
 
 import dns_ic
 import pyoracle
-from conftest import rel_l2, stats_errors
+from conftest import ftp_compare, rel_l2, stats_errors
 
 TOL = 1e-12
 
@@ -219,6 +219,42 @@ def test_plane_stats_mean_flow_closed_form():
     assert np.allclose(o["Um"][j], Mx * Mz * U[j], rtol=1e-15)
     assert np.allclose(o["U2m"][j], Mx * Mz * U[j] ** 2, rtol=1e-14)
     assert np.all(o["UVm"] == 0) and np.all(o["W2m"] == 0) and np.all(o["wx2m"] == 0)
+
+
+@pytest.mark.parametrize("case,nvar", [("A", False), ("B", True)])
+def test_oracle_four_to_phys_vs_reference_golden(golden, case, nvar):
+    """The post-processors four_to_phys (case A: record_out's header) and four_to_phys_N (case B: a
+    three-band header), compiled from the reference's sources, on random field files: every output
+    record (header records and plane headers byte for byte, planes to rounding)."""
+    g = golden("golden_ftp_16x16x9.npz")
+    files = {n: g["%s_in_%s" % (case, n)].tobytes() for n in ("u1", "u2", "u3", "p")}
+    out = pyoracle.four_to_phys_files(files, nvar)
+    for n in ("u1", "u2", "u3", "p"):
+        ftp_compare(out[n], g["%s_out_%s" % (case, n)].tobytes(), 1e-14)
+
+
+@pytest.mark.parametrize("Mx,Mz", [(24, 24), (16, 16), (24, 16)])
+def test_ftp_plane_single_modes_closed_form(Mx, Mz):
+    """One mode c at (kx, kz) gives 2 Re(c e^{2 pi i (kx x/Mx + kz z/Mz)}), and Re(c e^{2 pi i kz z/Mz})
+    at kx = 0 (the real x-inverse takes Re of that column); the kz = -Nz/2 row is dropped on the 3/2 grid
+    (Mz > Nz) and kept when Mz == Nz."""
+    Nx = Nz = 16
+    x = np.arange(Mx)[None, :]
+    z = np.arange(Mz)[:, None]
+    for kx, kz, c in ((1, 2, 1.0 + 0.5j), (0, 3, 0.7 - 0.2j), (5, -4, -0.3 + 1.1j), (2, -8, 1.0 + 0.0j)):
+        b = np.zeros((Nz, Nx + 2))
+        b[kz % Nz, 2 * kx] = c.real
+        b[kz % Nz, 2 * kx + 1] = c.imag
+        u = pyoracle.ftp_plane(b, Mx, Mz)
+        ph = np.exp(2j * np.pi * (kx * x / Mx + kz * z / Mz))
+        if kz == -Nz // 2 and Mz > Nz:
+            want = np.zeros((Mz, Mx))
+        elif kx == 0:
+            want = (c * ph).real
+        else:
+            want = 2 * (c * ph).real
+        assert np.allclose(u[:, :Mx], want, atol=1e-12), (kx, kz)
+        assert not np.any(u[:, Mx:])
 
 
 def test_planted_mistake_is_caught(golden):
