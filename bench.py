@@ -132,6 +132,29 @@ def traffic(kernel, prec_tag):
         return json.load(f).get("bytes_per_launch")
 
 
+def post_leg(u1):
+    """four_to_phys of the u1 field (384 x 384 modes, 259 planes) to the 576 x 576 grid on the GPU: kernel
+    time of dns_four_to_phys (event-timed, host copies outside) and its algorithmic HBM rate (input and
+    output planes once)."""
+    import numpy as np
+    import dnship
+    import dns_io
+    import dns_roofline
+    cx = dns_io.columns_to_planes(u1, 2 * MZ // 3)            # [planes, Nz, Kx] complex
+    planes = np.empty(cx.shape[:2] + (2 * cx.shape[2],))
+    planes[:, :, 0::2] = cx.real
+    planes[:, :, 1::2] = cx.imag
+    dnship.four_to_phys(planes[:2], MX, MZ)                    # load the kernels
+    _, ms = dnship.four_to_phys(planes, MX, MZ)
+    n, nz, nx = planes.shape
+    byts = n * (nx * nz + (MX + 2) * MZ) * 8.0
+    mid = n * 2 * (nx // 2) * MZ * 16.0
+    return {"planes": n, "ms": ms, "GBs": byts / (ms * 1e-3) / 1e9,
+            "frac_hbm": byts / (ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
+            "algorithmic_GB": byts / 1e9, "intermediate_GB": mid / 1e9,
+            "what": "u1 snapshot (%d planes of %d x %d modes) to %d x %d physical planes" % (n, nx - 2, nz, MX, MZ)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +163,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-post", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
@@ -195,6 +219,8 @@ def main():
         out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}
         if not args.no_mixed:
             out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
+    if not args.no_post:
+        out["four_to_phys"] = post_leg(f64[0])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(st)
     print(json.dumps(out), flush=True)

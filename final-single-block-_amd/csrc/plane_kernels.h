@@ -102,17 +102,20 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
 //   Z_n = (X_n + conj X_{H-n}) + i e^{+2 pi i n/MX} (X_n - conj X_{H-n}),  z = IFFT_H(Z)
 // Forward (rft_buff.f:62-74: scaled 1/MX, Im c_0 = 0):
 //   X_k = [ (Z_k + conj Z_{H-k}) - i e^{-2 pi i k/MX} (Z_k - conj Z_{H-k}) ] / (2 MX),  Z = FFT_H(z)
-// Each row owns 4 LDS slots: the 3 velocities go to slots 0..2, the products are formed in place
-// (u-planes: uu, uw, vv, ww in slots 0..3; v-planes: uv, wv in slots 0..1).  VPL selects the plane type;
-// the launch covers planes of one type only.
+// VPL selects the plane type (the launch covers planes of one type).  The 3 velocities go to the row's
+// slots 0..2 and the products are formed in place: u-planes uu, uw, vv, ww in 4 slots per row, v-planes
+// uv, wv in slots 0..1 of 3 slots per row (the smaller LDS footprint keeps more v-rows resident).
+
+template <bool VPL> constexpr int xpass_slots() { return VPL ? 3 : 4; }  // LDS slots per row
+
 template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
 __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   const PlaneArgs a = plane_view<R>(a_, blockIdx.y / a_.nbj);  // grid y = planes x ensemble members
   constexpr int H = MX / 2;
-  constexpr int NS = RB * 4;
-  constexpr int NP = VPL ? 2 : 4;
-  using LI = Lay<H, NS, RB * 3, RB, 4, PS, true, true>;  // inverse: active f*RB + r -> slot r*4 + f
-  using LF = Lay<H, NS, RB * NP, NP, 4, PS, true>;  // forward: slots r*4 + p, p < NP
+  constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
+  constexpr int NS = RB * NSL;
+  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true>;  // inverse: active f*RB + r -> slot r*NSL + f
+  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true>;            // forward: slots r*NSL + p, p < NP
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
   const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
@@ -148,7 +151,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   // each thread stays on one row (NTH/RB threads per row) and strides its positions: no division
   constexpr int TPR = NTH / RB;
   static_assert(NTH % RB == 0, "threads per row");
-  cx<R> *const prow = A + (tidx() / TPR) * 4 * LI::LR;
+  cx<R> *const prow = A + (tidx() / TPR) * NSL * LI::LR;
   for (int n = tidx() % TPR; n < H; n += TPR) {
     const int e = LI::row(n);
     cx<R> *b0 = prow + e;
@@ -182,18 +185,20 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   const R sc = (R)(0.5 / (double)MX);
   {  // each thread stays on one (row, product) pair and strides its modes: no division per output
     constexpr int SLOTS = RB * NP, TPS = NTH / SLOTS;
-    static_assert(NTH % SLOTS == 0, "threads per output row");
+    static_assert(TPS >= 1, "threads per output row");
     const int q = tidx() / TPS, r = q / NP, p = q - r * NP;
-    const cx<R> *b0 = A + (r * 4 + p) * LF::LR;
-    cx<R> *out = P + ((size_t)p * MZ + (z0 + r)) * Kp;
-    for (int k = tidx() % TPS; k < Kx; k += TPS) {
-      const cx<R> zk = b0[LF::row(k)];
-      const cx<R> zm = cconj(b0[LF::row(k == 0 ? 0 : H - k)]);
-      const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
-      cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
-      x = cscale(x, sc);
-      if (k == 0) x.y = 0;
-      out[k] = x;
+    if (q < SLOTS) {
+      const cx<R> *b0 = A + (r * NSL + p) * LF::LR;
+      cx<R> *out = P + ((size_t)p * MZ + (z0 + r)) * Kp;
+      for (int k = tidx() % TPS; k < Kx; k += TPS) {
+        const cx<R> zk = b0[LF::row(k)];
+        const cx<R> zm = cconj(b0[LF::row(k == 0 ? 0 : H - k)]);
+        const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
+        cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
+        x = cscale(x, sc);
+        if (k == 0) x.y = 0;
+        out[k] = x;
+      }
     }
   }
 }

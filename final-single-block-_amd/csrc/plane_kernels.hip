@@ -211,7 +211,7 @@ template <typename R, int MX, int V, bool VPL>
 static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
   using X = XTile<MX, V>;
   constexpr int H = MX / 2;
-  if constexpr (V > 0 && (size_t)X::RB * 4 * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
+  if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
     xpass_v<R, MX, 0, VPL>(a, nw, st);
   } else {
     if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // default tile: 4 rows divide Mz
