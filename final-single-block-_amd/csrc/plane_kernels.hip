@@ -161,13 +161,17 @@ template <> struct XPlanT<576> { using type = Plan<24, 24>; };
 
 constexpr size_t LDS_MAX = 160 * 1024;
 
-// z-kernel tiles (T columns, NTH threads); 12.12.x plans: 48 threads per column (one butterfly each).
+// z-kernel tiles (T columns, NTH threads, ZCH planes per z-inverse workgroup, which reads ZCH + 1 rows
+// of each field); 12.12.x plans: 48 threads per column (one butterfly each), at most 768 threads.
+// 576: variant 1 is variant 0 with 4 planes per z-inverse workgroup (fewer rows read twice); variant 2
+// takes 32 columns (two butterflies per thread; 512-byte FP32 row segments, an LDS tile that only FP32
+// fits, FP64 falling back to variant 0).
 template <int N, int V> struct ZTile {
   static constexpr bool p12 = (N == 576 || N == 1152);
-  static constexpr int T = N == 576 ? (V == 0 ? 16 : V == 1 ? 8 : 4)
+  static constexpr int T = N == 576 ? (V == 2 ? 32 : 16)
                          : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 8 : 4);
-  static constexpr int NTH = p12 ? 48 * T : (V == 2 ? 128 : 256);
-  static constexpr int ZCH = 2;  // planes per z-inverse workgroup (ZCH + 1 field rows each)
+  static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (V == 2 ? 128 : 256);
+  static constexpr int ZCH = (N == 576 && V == 1) ? 4 : 2;
 };
 // x-pass tiles (RB rows, NTH threads)
 template <int MX, int V> struct XTile {
