@@ -201,6 +201,13 @@ def main():
     ref64 = dns_roofline.reference_design_step_bytes(MX, MZ, NY, False)
     out["reference_design_step_GB"] = ref64 / 1e9
     out["step_frac_hbm_reference_design"] = ref64 / sec64 / 1e9 / dns_roofline.HBM_PEAK_GBS
+    cls = dns_roofline.reference_design_substep_classes(MX, MZ, NY, False)
+    plane_ms = sum(kms[k] for k in PLANE_KERNELS)
+    col_ms = sum(v for k, v in kms.items() if k not in PLANE_KERNELS)
+    out["class_frac_hbm_reference_design"] = {
+        "fft_and_transposes": cls["fft_and_transposes"] / (plane_ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
+        "column": cls["column"] / (col_ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
+        "plane_stage_ms_per_substep": plane_ms, "column_stage_ms_per_substep": col_ms}
     out["max_div"] = sc64["divmax"]
     out["autotune"] = {"fp64": tune64}
     if not args.no_mixed:
@@ -214,6 +221,12 @@ def main():
                         "step_frac_hbm_reference_design":
                             dns_roofline.reference_design_step_bytes(MX, MZ, NY, True) / secmx / 1e9
                             / dns_roofline.HBM_PEAK_GBS}
+        clm = dns_roofline.reference_design_substep_classes(MX, MZ, NY, True)
+        pm = sum(timmx[k] for k in PLANE_KERNELS)
+        cm = sum(v for k, v in timmx.items() if k not in PLANE_KERNELS)
+        out["mixed"]["class_frac_hbm_reference_design"] = {
+            "fft_and_transposes": clm["fft_and_transposes"] / (pm * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
+            "column": clm["column"] / (cm * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS}
         out["autotune"]["mixed"] = tunemx
     if not args.no_ensemble:
         out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}

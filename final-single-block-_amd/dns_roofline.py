@@ -64,6 +64,21 @@ def roofline(kernel: str, Mx: int, Mz: int, ny: int, fp32: bool, avg_ms: float) 
             "bytes_per_launch": by}
 
 
+def reference_design_substep_classes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> dict:
+    """The reference-design bytes of one substep split by kernel class: "fft_and_transposes" (the 12
+    transposes, z-inverse, x-pass and z-forward terms) and "column" (21.5 Cb); see
+    reference_design_step_bytes."""
+    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    Kx = Nx // 2 + 1
+    sf = 4 if fp32 else 8
+    S, I = Kx * Nz * 2 * sf, Kx * Mz * 2 * sf
+    Cb = Kx * Nz * ny * 16
+    nu, nv = ny - 1, ny - 2
+    fft = (2 * (3 * nu + 3 * nv) * 2 * S + (3 * nu + 3 * nv) * (S + I) + nu * 7 * I + nv * 5 * I
+           + nu * (4 * I + 3 * S) + nv * (2 * I + 3 * S))
+    return {"fft_and_transposes": float(fft), "column": 21.5 * Cb}
+
+
 def reference_design_step_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -> float:
     """Compulsory HBM bytes of one RK3 step of the reference's own decomposition (columns -> 12 transposes
     -> plane FFTs -> column solves), per substep:
@@ -72,12 +87,5 @@ def reference_design_step_bytes(Mx: int, Mz: int, ny: int, fp32: bool = False) -
     with S = Kx Nz 2s_f, I = Kx Mz 2s_f (s_f = 8 FP64, 4 mixed), Cb = Kx Nz ny 16, nu = ny - 1, nv = ny - 2.
     The device layout here needs no transposes, so its own compulsory bytes (kernel_bytes) are lower;
     this figure prices a step the way the reference's data movement would."""
-    Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
-    Kx = Nx // 2 + 1
-    sf = 4 if fp32 else 8
-    S, I = Kx * Nz * 2 * sf, Kx * Mz * 2 * sf
-    Cb = Kx * Nz * ny * 16
-    nu, nv = ny - 1, ny - 2
-    sub = (2 * (3 * nu + 3 * nv) * 2 * S + (3 * nu + 3 * nv) * (S + I) + nu * 7 * I + nv * 5 * I
-           + nu * (4 * I + 3 * S) + nv * (2 * I + 3 * S) + 21.5 * Cb)
-    return 3.0 * sub
+    c = reference_design_substep_classes(Mx, Mz, ny, fp32)
+    return 3.0 * (c["fft_and_transposes"] + c["column"])
