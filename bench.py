@@ -114,6 +114,14 @@ def cpu_leg(st):
         return {"value": 1.0 / (3.0 * t), "unit": "steps/s", "cores": 1, "kind": "reference",
                 "sample": "one RK3 substep (kRK=1) of the 384x384x257 workload, single MPI rank, %.1f s; "
                           "steps/s = 1/(3 x substep time)" % t}
+    return cpu_all_cores_leg(st)
+
+
+def cpu_all_cores_leg(st):
+    """The oracle's C restatement of the step on every host core the process may use (OpenMP), one RK3
+    step of the 384x384x257 workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
     n = len(os.sched_getaffinity(0))
     o = pyoracle.Oracle(MX, MZ, NY, Re=RE, mpgx=MPGX, threads=n)
     o.set_state(*st)
@@ -133,7 +141,7 @@ def traffic(kernel, prec_tag):
 
 
 def post_leg(u1):
-    """four_to_phys of the u1 field (384 x 384 modes, 259 planes) to the 576 x 576 grid on the GPU: kernel
+    """four_to_phys of the u1 field (384 x 384 modes, 258 planes) to the 576 x 576 grid on the GPU: kernel
     time of dns_four_to_phys (event-timed, host copies outside) and its algorithmic HBM rate (input and
     output planes once)."""
     import numpy as np
@@ -236,6 +244,8 @@ def main():
         out["four_to_phys"] = post_leg(f64[0])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(st)
+        if out["cpu_baseline"]["kind"] == "reference":
+            out["cpu_port_all_cores"] = cpu_all_cores_leg(st)
     print(json.dumps(out), flush=True)
 
 

@@ -352,6 +352,16 @@ template <int REM, int... Rs> struct greedy_plan<REM, Plan<Rs...>> {
 template <int... Rs> struct greedy_plan<1, Plan<Rs...>> { using type = Plan<Rs...>; };
 template <int N> using GreedyPlan = typename greedy_plan<N, Plan<>>::type;
 
+// Plans of the tuned lengths (chosen by timing on MI355X): z-transforms of 576 and 1152 as 12.12.x (48
+// threads per column give one butterfly per thread in every pass), the x half-lengths 288 and 576 in two
+// passes.  Other lengths take the greedy plan.
+template <int N> struct ZPlanT { using type = GreedyPlan<N>; };
+template <> struct ZPlanT<576> { using type = Plan<12, 12, 4>; };
+template <> struct ZPlanT<1152> { using type = Plan<12, 12, 8>; };
+template <int H> struct XPlanT { using type = GreedyPlan<H>; };
+template <> struct XPlanT<288> { using type = Plan<16, 18>; };
+template <> struct XPlanT<576> { using type = Plan<24, 24>; };
+
 // Runs the passes of plan PL starting at span P (LDS to LDS).
 template <typename R, class L, int NTH, int S, int P, class PL> struct fft_passes;
 template <typename R, class L, int NTH, int S, int P> struct fft_passes<R, L, NTH, S, P, Plan<>> {

@@ -152,12 +152,6 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
 #define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
 #define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
 
-template <int N> struct ZPlanT { using type = GreedyPlan<N>; };
-template <> struct ZPlanT<576> { using type = Plan<12, 12, 4>; };
-template <> struct ZPlanT<1152> { using type = Plan<12, 12, 8>; };
-template <int H> struct XPlanT { using type = GreedyPlan<H>; };
-template <> struct XPlanT<288> { using type = Plan<16, 18>; };
-template <> struct XPlanT<576> { using type = Plan<24, 24>; };
 
 constexpr size_t LDS_MAX = 160 * 1024;
 

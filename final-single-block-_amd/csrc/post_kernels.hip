@@ -16,15 +16,19 @@
 
 namespace dnsk {
 
-// z-tile: T columns of one plane per workgroup, T * MZ complex in LDS
+// z-tile: T columns of one plane per workgroup, T * MZ complex in LDS; the 12.12.x plans of 576 and
+// 1152 take 48 threads per column (one butterfly per thread and pass)
 template <int MZ> struct FtpZ {
   static constexpr int T = MZ * 32 <= 4608 ? 32 : MZ * 16 <= 4608 ? 16 : MZ * 8 <= 4608 ? 8 : 4;
+  static constexpr int NTH = (MZ == 576 || MZ == 1152) ? 48 * T : 256;
+};
+// x-tile: RB rows per workgroup
+template <int H> struct FtpX {
+  static constexpr int RB = 8;
   static constexpr int NTH = 256;
 };
-constexpr int FTP_RB = 8;   // rows per x-workgroup
-constexpr int FTP_XTH = 256;
 
-template <int MZ, int T, int NTH>
+template <int MZ, int T, int NTH, class PL>
 __global__ void __launch_bounds__(NTH) ftp_z_kernel(const double *__restrict__ in, double2 *__restrict__ mid,
                                                     const double2 *__restrict__ twz, int Nx, int Nz, int drop) {
   __shared__ double2 s[MZ * T];
@@ -38,7 +42,7 @@ __global__ void __launch_bounds__(NTH) ftp_z_kernel(const double *__restrict__ i
     s[idx] = v;
   }
   __syncthreads();
-  fft_lds<double, Lay<MZ, T>, NTH, +1>(s, twz);
+  fft_lds<double, Lay<MZ, T>, NTH, +1, PL>(s, twz);
   double2 *pm = mid + (size_t)pl * MZ * KX;
   for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
     const int z = idx / T, i = i0 + idx % T;
@@ -46,7 +50,7 @@ __global__ void __launch_bounds__(NTH) ftp_z_kernel(const double *__restrict__ i
   }
 }
 
-template <int H, int RB, int NTH>
+template <int H, int RB, int NTH, class PL>
 __global__ void __launch_bounds__(NTH) ftp_x_kernel(const double2 *__restrict__ mid, double *__restrict__ out,
                                                     const double2 *__restrict__ twx, const double2 *__restrict__ twh,
                                                     int KX, int MZ) {
@@ -71,7 +75,7 @@ __global__ void __launch_bounds__(NTH) ftp_x_kernel(const double2 *__restrict__ 
     s[n * RB + r] = cadd(sum, cmul_si<1>(cmul(dif, cconj(twx[n]))));
   }
   __syncthreads();
-  fft_lds<double, Lay<H, RB>, NTH, +1>(s, twh);
+  fft_lds<double, Lay<H, RB>, NTH, +1, PL>(s, twh);
   for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
     const int r = idx / H, n = idx % H, z = z0 + r;
     if (z < MZ) {
@@ -110,7 +114,8 @@ int launch_four_to_phys(const FtpArgs &a, hipStream_t st) {
 #define CASE(N)                                                                                        \
   if (a.Mz == N) {                                                                                     \
     using Z = FtpZ<N>;                                                                                 \
-    ftp_z_kernel<N, Z::T, Z::NTH><<<dim3((KX + Z::T - 1) / Z::T, a.n), Z::NTH, 0, st>>>(a.in, a.mid, a.twz, \
+    ftp_z_kernel<N, Z::T, Z::NTH, typename ZPlanT<N>::type>                                          \
+        <<<dim3((KX + Z::T - 1) / Z::T, a.n), Z::NTH, 0, st>>>(a.in, a.mid, a.twz,                     \
                                                                                       a.Nx, a.Nz, drop); \
     ok = true;                                                                                         \
   }
@@ -120,8 +125,9 @@ int launch_four_to_phys(const FtpArgs &a, hipStream_t st) {
   ok = false;
 #define CASE(N)                                                                                       \
   if (a.Mx / 2 == N) {                                                                                \
-    ftp_x_kernel<N, FTP_RB, FTP_XTH><<<dim3((a.Mz + FTP_RB - 1) / FTP_RB, a.n), FTP_XTH, 0, st>>>(    \
-        a.mid, a.out, a.twx, a.twh, KX, a.Mz);                                                        \
+    using X = FtpX<N>;                                                                                \
+    ftp_x_kernel<N, X::RB, X::NTH, typename XPlanT<N>::type>                                          \
+        <<<dim3((a.Mz + X::RB - 1) / X::RB, a.n), X::NTH, 0, st>>>(a.mid, a.out, a.twx, a.twh, KX, a.Mz); \
     ok = true;                                                                                        \
   }
   FTP_SIZES(CASE)
