@@ -140,6 +140,33 @@ def traffic(kernel, prec_tag):
         return json.load(f).get("bytes_per_launch")
 
 
+def stats_leg(st, reps=10):
+    """dns_accumulate_stats on the 384x384x257 state (plane statistics and spectra of stats.f90 and
+    spectra.f90, on the device): time per accumulation (host clock around synchronised calls) and its HBM
+    rate for the algorithmic bytes (u1, u2, u3, p read once; the five spectra read and written once)."""
+    import dnship
+    import dns_roofline
+    g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX)
+    g.upload(*st)
+    for _ in range(3):
+        g.accumulate_stats()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.accumulate_stats()
+    g.sync()
+    sec = (time.perf_counter() - t0) / reps
+    C = g.C
+    nyu, nyv = NY - 1, NY - 2
+    fields = (2 * (nyu + 2) + (nyv + 2) + nyu) * C * 16
+    spectra = 2 * (3 * (nyu + 2) + (nyv + 2) + nyu) * C * 8
+    g.close()
+    by = fields + spectra
+    return {"ms": sec * 1e3, "algorithmic_GB": by / 1e9, "GBs": by / sec / 1e9,
+            "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
+            "what": "one accumulation of the 13 plane statistics and 5 spectra at 384x384x257 (FP64)"}
+
+
 def post_leg(u1):
     """four_to_phys of the u1 field (384 x 384 modes, 258 planes) to the 576 x 576 grid on the GPU: kernel
     time of dns_four_to_phys (event-timed, host copies outside) and its algorithmic HBM rate (input and
@@ -242,6 +269,7 @@ def main():
             out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
     if not args.no_post:
         out["four_to_phys"] = post_leg(f64[0])
+        out["stats"] = stats_leg(st)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(st)
         if out["cpu_baseline"]["kind"] == "reference":

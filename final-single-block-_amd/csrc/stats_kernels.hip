@@ -19,7 +19,8 @@
 //     sum w^2 = Nx Nz Q'(w) with kx = 0 and kx = Nx/2 (the Nyquist of the unpadded x-transform) taking
 //     the half-terms and kz = -Nz/2 paired with itself.
 // Partial sums per chunk of columns go to a scratch array and are added in a fixed order, so the
-// results are deterministic.
+// results are deterministic.  One kernel reads every field row once and does both: the spectra of the
+// row's columns (read-modify-write) and the plane sums of its chunk.
 #include "dns_kernels.h"
 
 namespace dnsk {
@@ -84,6 +85,9 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
   b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp;
   const int nyu = a.nyu;
   const bool uin = j >= 1 && j <= nyu;          // u and v planes in physical space
+  const size_t C = a.C, nu = (size_t)(nyu + 2) * C, nv = (size_t)(a.nyv + 2) * C;
+  double *spU = a.sp + m * (3 * nu + nv + (size_t)nyu * C);
+  double *spV = spU + nu, *spW = spV + nv, *spUV = spW + nu, *spP = spUV + nu;
   double s[STAT_NQ];
 #pragma unroll
   for (int q = 0; q < STAT_NQ; q++) s[q] = 0.0;
@@ -94,6 +98,10 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
     {
       const double2 x1 = b.u1[(size_t)j * a.C + col], x3 = b.u3[(size_t)j * a.C + col];
       const double2 xv = u2_on_u(b, b.u2, col, j);
+      const size_t o = (size_t)j * C + col;         // spectra.f90:29-60
+      spU[o] = spU[o] + x1.x * x1.x + x1.y * x1.y;
+      spW[o] = spW[o] + x3.x * x3.x + x3.y * x3.y;
+      spUV[o] = spUV[o] + x1.x * xv.x + x1.y * xv.y;
       if (uin) {
         if (col == 0) { s[0] += x1.x; s[2] += x3.x; }
         if (i >= 1) {
@@ -118,6 +126,7 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
     if (j <= nyu) {
       const double2 x2 = b.u2[(size_t)j * a.C + col];
       const double2 xw = u_on_v(b, b.u3, col, j);
+      spV[(size_t)j * C + col] = spV[(size_t)j * C + col] + x2.x * x2.x + x2.y * x2.y;
       if (uin) {
         if (col == 0) s[6] += x2.x;
         if (i >= 1) {
@@ -148,6 +157,7 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
     // ---- p planes 1..nyu: spectral layout
     if (uin) {
       const double2 x = b.p[(size_t)(j - 1) * a.C + col];
+      spP[(size_t)(j - 1) * C + col] = spP[(size_t)(j - 1) * C + col] + x.x * x.x + x.y * x.y;
       s[11] += sum_ri(x);
       s[12] += re_dot(x, x);
     }
@@ -183,41 +193,9 @@ __global__ void stats_finish_kernel(StatsArgs a) {
   a.acc[((size_t)m * STAT_NQ + q) * nj + j] += f * v;
 }
 
-// spectra.f90:29-60, thread per column
-__global__ void spectra_kernel(StatsArgs a) {
-  const size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
-  if (col >= (size_t)a.C) return;
-  const MemberStrides ms = member_strides(a.C, a.nyu, a.nyv);
-  StatsArgs b = a;
-  b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp;
-  const size_t C = a.C, nu = (size_t)(a.nyu + 2) * C, nv = (size_t)(a.nyv + 2) * C;
-  double *spU = a.sp + m * (3 * nu + nv + (size_t)a.nyu * C);
-  double *spV = spU + nu, *spW = spV + nv, *spUV = spW + nu, *spP = spUV + nu;
-  for (int j = 0; j <= a.nyu + 1; j++) {
-    const size_t o = (size_t)j * C + col;
-    const double2 x1 = b.u1[o], x3 = b.u3[o];
-    spU[o] = spU[o] + x1.x * x1.x + x1.y * x1.y;
-    spW[o] = spW[o] + x3.x * x3.x + x3.y * x3.y;
-    const double2 xv = u2_on_u(b, b.u2, col, j);
-    spUV[o] = spUV[o] + x1.x * xv.x + x1.y * xv.y;
-  }
-  for (int j = 0; j <= a.nyv + 1; j++) {
-    const size_t o = (size_t)j * C + col;
-    const double2 x = b.u2[o];
-    spV[o] = spV[o] + x.x * x.x + x.y * x.y;
-  }
-  for (int j = 1; j <= a.nyu; j++) {
-    const size_t o = (size_t)(j - 1) * C + col;
-    const double2 x = b.p[o];
-    spP[o] = spP[o] + x.x * x.x + x.y * x.y;
-  }
-}
-
 int stats_nchunk(int C) { return (C + STAT_CHUNK - 1) / STAT_CHUNK; }
 
 void launch_stats(const StatsArgs &a, hipStream_t st) {
-  spectra_kernel<<<dim3((a.C + 255) / 256, a.nm), 256, 0, st>>>(a);
   plane_stats_kernel<<<dim3(a.nchunk, a.nyu + 2, a.nm), STAT_NTH, 0, st>>>(a);
   const int n = a.nm * (a.nyu + 2) * STAT_NQ;
   stats_finish_kernel<<<(n + 255) / 256, 256, 0, st>>>(a);
