@@ -1008,10 +1008,13 @@ extern "C" int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, con
                 "8..1152, 2^a 3^b)", Mx, Mz);
   if (ms) *ms = 0.0;
   if (nplanes == 0) return DNS_OK;
-  const size_t KX = Nx / 2 + 1, nin = (size_t)(Nx + 2) * Nz, nout = (size_t)(Mx + 2) * Mz, nmid = KX * Mz;
-  // planes per batch: at most ~1 GiB of device buffers
+  // device rows padded to whole 128-byte lines (the host copies re-pitch the rows)
+  const int ip = (Nx + 2 + 15) / 16 * 16, op = (Mx + 2 + 15) / 16 * 16, mp = (Nx / 2 + 1 + 7) / 8 * 8;
+  const size_t nin = (size_t)ip * Nz, nout = (size_t)op * Mz, nmid = (size_t)mp * Mz;
+  const size_t hin = (size_t)(Nx + 2) * Nz, hout = (size_t)(Mx + 2) * Mz;   // host plane sizes
+  // planes per batch: at most ~4 GiB of device buffers
   const size_t per = (nin + nout) * sizeof(double) + nmid * sizeof(double2);
-  const int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nplanes, ((size_t)1 << 30) / per));
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nplanes, ((size_t)4 << 30) / per));
   double *din = nullptr, *dout = nullptr;
   double2 *dmid = nullptr, *tw = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1032,9 +1035,10 @@ extern "C" int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, con
     double total = 0.0;
     for (int p0 = 0; p0 < nplanes && rc == DNS_OK; p0 += nb) {
       const int n = std::min(nb, nplanes - p0);
-      if (!hip_ok(hipMemcpy(din, in + (size_t)p0 * nin, n * nin * sizeof(double), hipMemcpyHostToDevice), "upload"))
+      if (!hip_ok(hipMemcpy2D(din, ip * sizeof(double), in + (size_t)p0 * hin, (Nx + 2) * sizeof(double),
+                              (Nx + 2) * sizeof(double), (size_t)n * Nz, hipMemcpyHostToDevice), "upload"))
         break;
-      dnsk::FtpArgs fa{Nx, Nz, Mx, Mz, n, din, dmid, dout, tw, tw + Mz, tw + Mz + Mx};
+      dnsk::FtpArgs fa{Nx, Nz, Mx, Mz, n, ip, mp, op, din, dmid, dout, tw, tw + Mz, tw + Mz + Mx};
       (void)hipEventRecord(e0, 0);
       if (dnsk::launch_four_to_phys(fa, 0)) {
         rc = fail(DNS_ERR_ARG, "dns_four_to_phys: %d x %d not instantiated", Mx, Mz);
@@ -1042,8 +1046,8 @@ extern "C" int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, con
       }
       (void)hipEventRecord(e1, 0);
       if (!hip_ok(hipGetLastError(), "launch") ||
-          !hip_ok(hipMemcpy(out + (size_t)p0 * nout, dout, n * nout * sizeof(double), hipMemcpyDeviceToHost),
-                  "download"))
+          !hip_ok(hipMemcpy2D(out + (size_t)p0 * hout, (Mx + 2) * sizeof(double), dout, op * sizeof(double),
+                              (Mx + 2) * sizeof(double), (size_t)n * Mz, hipMemcpyDeviceToHost), "download"))
         break;
       float t = 0.0f;
       (void)hipEventElapsedTime(&t, e0, e1);

@@ -191,9 +191,10 @@ void launch_stats(const StatsArgs &a, hipStream_t st);
 // four_to_phys post-processing (post_kernels.hip): n spectral planes (Nx+2) x Nz -> physical (Mx+2) x Mz
 struct FtpArgs {
   int Nx, Nz, Mx, Mz, n;
-  const double *in;
-  double2 *mid;      // z-inverted columns [plane][Mz][Nx/2+1]
-  double *out;
+  int in_pitch, mid_pitch, out_pitch;  // row pitches (doubles, complex, doubles): whole 128-byte lines
+  const double *in;  // [plane][Nz][in_pitch]
+  double2 *mid;      // z-inverted columns [plane][Mz][mid_pitch]
+  double *out;       // [plane][Mz][out_pitch]
   const double2 *twz, *twx, *twh;  // e^{-2 pi i m/N} for N = Mz, Mx, Mx/2
 };
 bool four_to_phys_supported(int Mx, int Mz);

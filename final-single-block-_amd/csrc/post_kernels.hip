@@ -16,13 +16,13 @@
 
 namespace dnsk {
 
-// z-tile: T columns of one plane per workgroup, T * MZ complex in LDS; the 12.12.x plans of 576 and
-// 1152 take 48 threads per column (one butterfly per thread and pass)
+// z-tile: T columns of one plane per workgroup (256-byte row segments where LDS allows), T * MZ complex
+// in LDS; the 12.12.x plans of 576 and 1152 take 48 threads per column (one butterfly per thread and pass)
 template <int MZ> struct FtpZ {
-  static constexpr int T = MZ * 32 <= 4608 ? 32 : MZ * 16 <= 4608 ? 16 : MZ * 8 <= 4608 ? 8 : 4;
+  static constexpr int T = MZ * 32 * 16 <= 150000 ? 32 : MZ * 16 * 16 <= 150000 ? 16 : 8;
   static constexpr int NTH = (MZ == 576 || MZ == 1152) ? 48 * T : 256;
 };
-// x-tile: RB rows per workgroup
+// x-tile: RB rows per workgroup (8 rows and 256 threads measured best at 288 points)
 template <int H> struct FtpX {
   static constexpr int RB = 8;
   static constexpr int NTH = 256;
@@ -30,39 +30,40 @@ template <int H> struct FtpX {
 
 template <int MZ, int T, int NTH, class PL>
 __global__ void __launch_bounds__(NTH) ftp_z_kernel(const double *__restrict__ in, double2 *__restrict__ mid,
-                                                    const double2 *__restrict__ twz, int Nx, int Nz, int drop) {
+                                                    const double2 *__restrict__ twz, int Nx, int Nz, int drop,
+                                                    int ip, int mp) {
   __shared__ double2 s[MZ * T];
   const int KX = Nx / 2 + 1, i0 = blockIdx.x * T, pl = blockIdx.y;
-  const double *pin = in + (size_t)pl * (Nx + 2) * Nz;
+  const double *pin = in + (size_t)pl * ip * Nz;
   for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
     const int z = idx / T, i = i0 + idx % T;
     const int r = z < Nz / 2 ? z : (z >= MZ - Nz / 2 + drop ? z + Nz - MZ : -1);
     double2 v = d2(0.0, 0.0);
-    if (i < KX && r >= 0) v = d2(pin[(size_t)r * (Nx + 2) + 2 * i], pin[(size_t)r * (Nx + 2) + 2 * i + 1]);
+    if (i < KX && r >= 0) v = reinterpret_cast<const double2 *>(pin + (size_t)r * ip)[i];  // ip is even
     s[idx] = v;
   }
   __syncthreads();
   fft_lds<double, Lay<MZ, T>, NTH, +1, PL>(s, twz);
-  double2 *pm = mid + (size_t)pl * MZ * KX;
+  double2 *pm = mid + (size_t)pl * MZ * mp;
   for (int idx = threadIdx.x; idx < MZ * T; idx += NTH) {
     const int z = idx / T, i = i0 + idx % T;
-    if (i < KX) pm[(size_t)z * KX + i] = s[idx];
+    if (i < KX) pm[(size_t)z * mp + i] = s[idx];
   }
 }
 
 template <int H, int RB, int NTH, class PL>
 __global__ void __launch_bounds__(NTH) ftp_x_kernel(const double2 *__restrict__ mid, double *__restrict__ out,
                                                     const double2 *__restrict__ twx, const double2 *__restrict__ twh,
-                                                    int KX, int MZ) {
+                                                    int KX, int MZ, int mp, int op) {
   __shared__ double2 s[H * RB];
   const int z0 = blockIdx.x * RB, pl = blockIdx.y;
-  const double2 *pm = mid + (size_t)pl * MZ * KX;
-  double *po = out + (size_t)pl * MZ * (2 * H + 2);
+  const double2 *pm = mid + (size_t)pl * MZ * mp;
+  double *po = out + (size_t)pl * MZ * op;
   for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
     const int r = idx / H, n = idx % H, z = z0 + r;
     double2 xa = d2(0.0, 0.0), xb = d2(0.0, 0.0);
     if (z < MZ) {
-      const double2 *row = pm + (size_t)z * KX;
+      const double2 *row = pm + (size_t)z * mp;
       const double2 cn = n < KX ? row[n] : d2(0.0, 0.0);
       xa = d2(cn.x, n == 0 ? 0.0 : cn.y);
       if (n == 0) {
@@ -79,18 +80,14 @@ __global__ void __launch_bounds__(NTH) ftp_x_kernel(const double2 *__restrict__ 
   for (int idx = threadIdx.x; idx < H * RB; idx += NTH) {
     const int r = idx / H, n = idx % H, z = z0 + r;
     if (z < MZ) {
-      const double2 v = s[n * RB + r];
-      po[(size_t)z * (2 * H + 2) + 2 * n] = v.x;
-      po[(size_t)z * (2 * H + 2) + 2 * n + 1] = v.y;
+      reinterpret_cast<double2 *>(po + (size_t)z * op)[n] = s[n * RB + r];   // op is even
     }
   }
   // entries Mx, Mx+1: the row's kx = Mx/2 coefficient, untouched by the x-transform
   if (threadIdx.x < RB) {
     const int z = z0 + threadIdx.x;
     if (z < MZ) {
-      const double2 c = H < KX ? pm[(size_t)z * KX + H] : d2(0.0, 0.0);
-      po[(size_t)z * (2 * H + 2) + 2 * H] = c.x;
-      po[(size_t)z * (2 * H + 2) + 2 * H + 1] = c.y;
+      reinterpret_cast<double2 *>(po + (size_t)z * op)[H] = H < KX ? pm[(size_t)z * mp + H] : d2(0.0, 0.0);
     }
   }
 }
@@ -115,8 +112,8 @@ int launch_four_to_phys(const FtpArgs &a, hipStream_t st) {
   if (a.Mz == N) {                                                                                     \
     using Z = FtpZ<N>;                                                                                 \
     ftp_z_kernel<N, Z::T, Z::NTH, typename ZPlanT<N>::type>                                          \
-        <<<dim3((KX + Z::T - 1) / Z::T, a.n), Z::NTH, 0, st>>>(a.in, a.mid, a.twz,                     \
-                                                                                      a.Nx, a.Nz, drop); \
+        <<<dim3((KX + Z::T - 1) / Z::T, a.n), Z::NTH, 0, st>>>(a.in, a.mid, a.twz, a.Nx, a.Nz, drop,   \
+                                                               a.in_pitch, a.mid_pitch);               \
     ok = true;                                                                                         \
   }
   FTP_SIZES(CASE)
@@ -127,7 +124,8 @@ int launch_four_to_phys(const FtpArgs &a, hipStream_t st) {
   if (a.Mx / 2 == N) {                                                                                \
     using X = FtpX<N>;                                                                                \
     ftp_x_kernel<N, X::RB, X::NTH, typename XPlanT<N>::type>                                          \
-        <<<dim3((a.Mz + X::RB - 1) / X::RB, a.n), X::NTH, 0, st>>>(a.mid, a.out, a.twx, a.twh, KX, a.Mz); \
+        <<<dim3((a.Mz + X::RB - 1) / X::RB, a.n), X::NTH, 0, st>>>(a.mid, a.out, a.twx, a.twh, KX, a.Mz, \
+                                                                   a.mid_pitch, a.out_pitch);           \
     ok = true;                                                                                        \
   }
   FTP_SIZES(CASE)
