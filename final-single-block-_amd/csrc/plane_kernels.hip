@@ -167,10 +167,10 @@ template <int N, int V> struct ZTile {
   static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (V == 2 ? 128 : 256);
   static constexpr int ZCH = (N == 576 && V == 1) ? 4 : 2;
 };
-// x-pass tiles (RB rows, NTH threads)
+// x-pass tiles (RB rows, NTH threads); variant 2: one wavefront per row (its barriers are wave-local)
 template <int MX, int V> struct XTile {
-  static constexpr int RB = V == 0 ? 4 : V == 1 ? 2 : 8;
-  static constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 512;
+  static constexpr int RB = V == 0 ? 4 : V == 1 ? 2 : 1;
+  static constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 64;
 };
 
 bool plane_size_supported(int n) {
