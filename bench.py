@@ -38,9 +38,10 @@ def kernel_times(tim, path):
     return k
 
 
-def gpu_leg(st, prec, steps, warmup):
+def gpu_leg(st, prec, steps, warmup, size=None, Re=RE, mpgx=MPGX):
     import dnship
-    g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX, precision=prec)
+    Mx, Mz, ny = size or (MX, MZ, NY)
+    g = dnship.DNS(Mx, Mz, ny, Re=Re, mpgx=mpgx, precision=prec)
     g.upload(*st)
     chosen, ms = g.autotune(iters=3)   # tile variants of the plane kernels, timed on this workload
     g.upload(*st)
@@ -140,6 +141,39 @@ def traffic(kernel, prec_tag):
         return json.load(f).get("bytes_per_launch")
 
 
+# The largest configuration: 768 x 384 Fourier modes x 385 y points (1152 x 576 dealiased), Re_tau ~ 1000
+LARGE = (1152, 576, 385)
+LARGE_RE = 19800.0
+LARGE_MPGX = -(1000.0 / LARGE_RE) ** 2
+
+
+def large_leg(steps, warmup):
+    """FP64 and mixed steps/s at 768x384x385 (seeded broadband IC, seed 5), with the HBM fractions of the
+    step's own and reference-design bytes and the mixed run's error against FP64."""
+    import numpy as np
+    import dnship
+    import dns_ic
+    import dns_roofline
+    st = dns_ic.seeded_channel(*LARGE, seed=5, amp=0.1, broadband=True)
+    out = {"workload": "RK3 step, 768x384 Fourier modes x 385 y points (1152x576 dealiased), Re=19800, "
+                       "Re_tau~1000"}
+    res = {}
+    for tag, prec in (("fp64", dnship.PREC_FP64), ("mixed", dnship.PREC_MIXED)):
+        sec, tim, sc, f, _ = gpu_leg(st, prec, steps, warmup, LARGE, LARGE_RE, LARGE_MPGX)
+        fp32 = prec == dnship.PREC_MIXED
+        by = dns_roofline.kernel_bytes(*LARGE, fp32)["step"]
+        ref = dns_roofline.reference_design_step_bytes(*LARGE, fp32)
+        out[tag] = {"value": 1.0 / sec, "ms_per_step": sec * 1e3, "kernels_ms": tim,
+                    "step_frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
+                    "step_frac_hbm_reference_design": ref / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
+                    "max_div": sc["divmax"]}
+        res[tag] = f
+    out["mixed"]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out["mixed"]["rel_l2_vs_fp64"] = {k: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                                      for k, a, b in zip(("u1", "u2", "u3"), res["mixed"], res["fp64"])}
+    return out
+
+
 def stats_leg(st, reps=10):
     """dns_accumulate_stats on the 384x384x257 state (plane statistics and spectra of stats.f90 and
     spectra.f90, on the device): time per accumulation (host clock around synchronised calls) and its HBM
@@ -199,6 +233,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-post", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
@@ -267,6 +302,8 @@ def main():
         out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}
         if not args.no_mixed:
             out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
+    if not args.no_large:
+        out["large"] = large_leg(max(1, min(args.steps, 5)), 1)
     if not args.no_post:
         out["four_to_phys"] = post_leg(f64[0])
         out["stats"] = stats_leg(st)

@@ -167,10 +167,13 @@ template <int N, int V> struct ZTile {
   static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (V == 2 ? 128 : 256);
   static constexpr int ZCH = (N == 576 && V == 1) ? 4 : 2;
 };
-// x-pass tiles (RB rows, NTH threads); variant 2: one wavefront per row (its barriers are wave-local)
+// x-pass tiles (RB rows, NTH threads).  Up to 576 points: 4, 2 or 1 rows per workgroup with one wavefront
+// per row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only
+// four rows fit a CU): one row per workgroup with 2, 4 or 1 wavefronts, for more wavefronts per CU.
 template <int MX, int V> struct XTile {
-  static constexpr int RB = V == 0 ? 4 : V == 1 ? 2 : 1;
-  static constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 64;
+  static constexpr bool big = MX >= 1152;
+  static constexpr int RB = big ? 1 : (V == 0 ? 4 : V == 1 ? 2 : 1);
+  static constexpr int NTH = big ? (V == 0 ? 128 : V == 1 ? 256 : 64) : (V == 0 ? 256 : V == 1 ? 128 : 64);
 };
 
 bool plane_size_supported(int n) {
@@ -180,7 +183,7 @@ bool plane_size_supported(int n) {
   return false;
 }
 
-int xpass_rows_per_block(int Mx) { (void)Mx; return 4; }  // rows of the default x-pass tile
+int xpass_rows_per_block(int Mx) { return Mx >= 1152 ? 1 : 4; }  // rows of the default x-pass tile
 
 int plane_variants(int n) { return DNS_TUNED(n) ? 3 : 1; }
 
@@ -212,7 +215,7 @@ static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
   if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
     xpass_v<R, MX, 0, VPL>(a, nw, st);
   } else {
-    if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // default tile: 4 rows divide Mz
+    if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // the default tile's rows divide Mz
     dim3 g(a.Mz / X::RB, nw * a.nm);
     using PL = typename XPlanT<H>::type;
     xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);

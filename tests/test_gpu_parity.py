@@ -248,6 +248,20 @@ def test_modes_planes_transposes_exact():
     h.close()
 
 
+def test_full_size_properties_768x384x385():
+    """The largest configuration (768 x 384 modes x 385 points, 1152 x 576 dealiased) FP64: one step; max|div|
+    at rounding level, and the mixed-precision step within 1e-6 of FP64."""
+    Mx, Mz, ny = 1152, 576, 385
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=5, broadband=True)
+    print("case: 1152x576x385 fp64 and mixed, one step", flush=True)
+    a, _, sa = gpu_run(Mx, Mz, ny, st, 1, Re=19800.0, mpgx=-(1000.0 / 19800.0) ** 2)
+    assert sa["divmax"] < 1e-8
+    b, _, _ = gpu_run(Mx, Mz, ny, st, 1, Re=19800.0, mpgx=-(1000.0 / 19800.0) ** 2,
+                      precision=dnship.PREC_MIXED)
+    for x, y in zip(a[:3], b[:3]):
+        assert rel_l2(y, x) <= 1e-6
+
+
 def test_full_size_properties_384():
     """384 x 384 x 257 FP64: one step; the projection leaves max|div| at rounding level and the
     mixed-precision step stays within 1e-6 of FP64 after one step."""
