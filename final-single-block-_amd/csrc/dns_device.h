@@ -269,6 +269,15 @@ struct Lay {
   }
 };
 
+// Workgroup b of nblk (linear order) as the hardware deals them out: to the 8 XCDs in turn, b % 8.  The
+// returned logical block gives each XCD a contiguous run of logical blocks (the first nblk % 8 XCDs one
+// more), so workgroups that run together on one XCD are neighbours in logical order and share the
+// 128-byte lines at their edges through that XCD's L2.  A bijection on 0..nblk-1.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Thread index, opaque to the optimiser: index arithmetic derived from it is recomputed where it is used
 // instead of being hoisted out of outer loops and kept live in registers across them.
 __device__ __forceinline__ int tidx() {

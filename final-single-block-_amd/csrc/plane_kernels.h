@@ -35,10 +35,15 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
   __shared__ cx<R> s[L::SIZE];
   constexpr int KS = last_slots<L, NTH, PL>();
   const int nch = (a_.nbj + ZCH - 1) / ZCH;  // grid z = chunks x ensemble members
-  const PlaneArgs a = plane_view<R>(a_, blockIdx.z / nch);
-  const int f = blockIdx.y;
-  const int jc0 = a.j0 + (blockIdx.z % nch) * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
-  const int i0 = blockIdx.x * T;
+  // logical block (tile fastest, then field, then chunk), XCD-aware: neighbouring tiles of one row run
+  // on one XCD and share the lines at their edges (a k-row of Kx modes is not line-aligned)
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+  const PlaneArgs a = plane_view<R>(a_, bz / nch);
+  const int f = by;
+  const int jc0 = a.j0 + (bz % nch) * ZCH, jc1 = min(jc0 + ZCH - 1, a.j0 + a.nbj - 1);
+  const int i0 = bx * T;
   const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);
   const size_t fsz = (size_t)MZ * Kp;
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
@@ -212,13 +217,17 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
 // fluxes keep it.
 template <typename R, int MZ, int T, int NTH, bool VPL, class PL, int WPE = 1, int PS = 4>
 __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a_) {
-  const PlaneArgs a = plane_view<R>(a_, blockIdx.y / a_.nbj);  // grid y = planes x ensemble members
+  // logical block (tile fastest), XCD-aware: neighbouring tiles share the output lines at their edges
+  const int gx = gridDim.x;
+  const int lb = xcd_block(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  const int bx = lb % gx, by = lb / gx;
+  const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
   using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
   __shared__ cx<R> s[L::SIZE];
   constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
   constexpr int NP = VPL ? 2 : 4;
-  const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
-  const int i0 = blockIdx.x * T;
+  const int jj = by % a.nbj, j = a.j0 + jj;
+  const int i0 = bx * T;
   constexpr int Nz = 2 * MZ / 3;    // retained z-modes (the launcher checks a.Nz)
   const int Kx = a.Kx, C = a.C, Kp = row_pitch<R>(Kx);
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.twz);
