@@ -174,6 +174,30 @@ def large_leg(steps, warmup):
     return out
 
 
+def error_leg(st, nsteps):
+    """Relative L2 velocity error of the mixed-precision path against FP64 after nsteps steps from the same
+    state (each path in its own handle, CFL time step; not timed)."""
+    import numpy as np
+    import dnship
+    out = {}
+    for tag, prec in (("fp64", dnship.PREC_FP64), ("mixed", dnship.PREC_MIXED)):
+        g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX, precision=prec)
+        g.upload(*st)
+        done = 0
+        while done < nsteps:          # in chunks, so a long run reports progress
+            n = min(100, nsteps - done)
+            g.step(n)
+            g.sync()
+            done += n
+            print("error leg: %s %d/%d steps" % (tag, done, nsteps), file=sys.stderr, flush=True)
+        out[tag] = (g.download(), g.scalars())
+        g.close()
+    (a, sa), (b, sb) = out["fp64"], out["mixed"]
+    return {"steps": nsteps, "t_fp64": sa["t"], "t_mixed": sb["t"],
+            "rel_l2": {k: float(np.linalg.norm(y - x) / np.linalg.norm(x))
+                       for k, x, y in zip(("u1", "u2", "u3"), a[:3], b[:3])}}
+
+
 def stats_leg(st, reps=10):
     """dns_accumulate_stats on the 384x384x257 state (plane statistics and spectra of stats.f90 and
     spectra.f90, on the device): time per accumulation (host clock around synchronised calls) and its HBM
@@ -234,6 +258,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-post", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=0,
+                    help="also report the mixed path's error against FP64 after this many steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
@@ -298,6 +324,8 @@ def main():
             "fft_and_transposes": clm["fft_and_transposes"] / (pm * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
             "column": clm["column"] / (cm * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS}
         out["autotune"]["mixed"] = tunemx
+        if args.error_steps > 0:
+            out["mixed"]["long_run_error"] = error_leg(st, args.error_steps)
     if not args.no_ensemble:
         out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}
         if not args.no_mixed:
