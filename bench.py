@@ -87,13 +87,13 @@ def ensemble_leg(steps, warmup, prec):
         g.autotune(iters=3)
         handles.append(g)
         by += nm * dns_roofline.kernel_bytes(Mx, Mz, ny, prec == dnship.PREC_MIXED)["step"]
-    for g in handles:
-        g.step(warmup)
+    for g in handles:          # each handle's RK3 step is captured once in a hipGraph and replayed
+        g.step_graph(max(warmup, 1))
     for g in handles:
         g.sync()
     t0 = time.perf_counter()
     for g in handles:          # one stream per handle: the three groups run concurrently
-        g.step(steps)
+        g.step_graph(steps)
     for g in handles:
         g.sync()
     sec = (time.perf_counter() - t0) / steps

@@ -119,7 +119,8 @@ int dns_meanflow_ctP(dns_handle *h);          /* meanflow_ctP   littleharsh.f90:
 
 /* ---- fused path ---- */
 /* One RK3 substep: RHS0 + nonlinear + solveU + meanflow_ctU + solveP + v_corr fused into the plane
- * stage (3 kernels) and the column stage (3 sweeps).  littleharsh.f90:165-222 */
+ * stage (3 kernels) and the column stage (one kernel, or 3 sweeps: dns_set_column_path).
+ * littleharsh.f90:165-222 */
 int dns_rk3_substep(dns_handle *h, int kRK);
 /* nsteps full RK3 steps (littleharsh.f90:163-233), including meanflow_ctP and iter += 1. */
 int dns_rk3_step(dns_handle *h, int nsteps);
