@@ -146,7 +146,7 @@ __device__ __forceinline__ ColArgs col_view(ColArgs a, int m) {
 }
 
 bool plane_size_supported(int n);
-int xpass_rows_per_block(int Mx);  // rows of one x-pass block (largest variant); must divide Mz
+int xpass_rows_per_block(int Mx);  // rows an x-pass launch needs Mz to be a multiple of (1: any Mz)
 int plane_variants(int n);          // tile variants instantiated for transform length n
 // which: 0 z-inverse (nw = nbj + 1 field rows), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes (nw = nbj)
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);

@@ -160,20 +160,28 @@ constexpr size_t LDS_MAX = 160 * 1024;
 // 576: variant 1 is variant 0 with 4 planes per z-inverse workgroup (fewer rows read twice); variant 2
 // takes 32 columns (two butterflies per thread; 512-byte FP32 row segments, an LDS tile that only FP32
 // fits, FP64 falling back to variant 0).
+// Other lengths: 16, 8 or 32 columns, with threads for about one butterfly of the 16-point first pass
+// each (64 to 256).
 template <int N, int V> struct ZTile {
   static constexpr bool p12 = (N == 576 || N == 1152);
   static constexpr int T = N == 576 ? (V == 2 ? 32 : 16)
-                         : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 8 : 4);
-  static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (V == 2 ? 128 : 256);
+                         : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 16 : V == 1 ? 8 : 32);
+  static constexpr int NB = (T * N / 16 + 63) / 64 * 64;  // first-pass butterflies, whole wavefronts
+  static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (NB < 64 ? 64 : NB > 256 ? 256 : NB);
   static constexpr int ZCH = (N == 576 && V == 1) ? 4 : 2;
 };
-// x-pass tiles (RB rows, NTH threads).  Up to 576 points: 4, 2 or 1 rows per workgroup with one wavefront
-// per row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only
-// four rows fit a CU): one row per workgroup with 2, 4 or 1 wavefronts, for more wavefronts per CU.
+// x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront per
+// row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only four
+// rows fit a CU): one row per workgroup with 2, 4 or 1 wavefronts.  Shorter rows have too few butterflies
+// per pass for a wavefront, so RPW rows share one: workgroups of 2, 1 or 4 wavefronts.
 template <int MX, int V> struct XTile {
-  static constexpr bool big = MX >= 1152;
-  static constexpr int RB = big ? 1 : (V == 0 ? 4 : V == 1 ? 2 : 1);
-  static constexpr int NTH = big ? (V == 0 ? 128 : V == 1 ? 256 : 64) : (V == 0 ? 256 : V == 1 ? 128 : 64);
+  static constexpr int H = MX / 2;
+  static constexpr int RPW = H <= 48 ? 4 : H <= 96 ? 2 : 1;   // rows per wavefront below 576 points
+  static constexpr bool big = MX >= 1152, mid = MX == 576;
+  static constexpr int RB = big ? 1 : mid ? (V == 0 ? 4 : V == 1 ? 2 : 1)
+                                         : (V == 0 ? 2 * RPW : V == 1 ? RPW : 4 * RPW);
+  static constexpr int NTH = big ? (V == 0 ? 128 : V == 1 ? 256 : 64) : mid ? (V == 0 ? 256 : V == 1 ? 128 : 64)
+                                                                          : (V == 0 ? 128 : V == 1 ? 64 : 256);
 };
 
 bool plane_size_supported(int n) {
@@ -183,7 +191,7 @@ bool plane_size_supported(int n) {
   return false;
 }
 
-int xpass_rows_per_block(int Mx) { return Mx >= 1152 ? 1 : 4; }  // rows of the default x-pass tile
+int xpass_rows_per_block(int Mx) { (void)Mx; return 1; }  // every Mz: tiles that do not divide it fall back
 
 int plane_variants(int n) { return DNS_TUNED(n) ? 3 : 1; }
 
@@ -215,9 +223,12 @@ static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
   if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
     xpass_v<R, MX, 0, VPL>(a, nw, st);
   } else {
-    if (a.Mz % X::RB != 0) return xpass_v<R, MX, 0, VPL>(a, nw, st);  // the default tile's rows divide Mz
-    dim3 g(a.Mz / X::RB, nw * a.nm);
     using PL = typename XPlanT<H>::type;
+    if (a.Mz % X::RB != 0) {  // rows that do not fill the tile: one row per single-wavefront workgroup
+      xpass_kernel<R, MX, 1, 64, VPL, PL, PL, 1, 4><<<dim3(a.Mz, nw * a.nm), 64, 0, st>>>(a);
+      return;
+    }
+    dim3 g(a.Mz / X::RB, nw * a.nm);
     xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);
   }
 }
