@@ -370,6 +370,8 @@ template <> struct ZPlanT<1152> { using type = Plan<12, 12, 8>; };
 template <int H> struct XPlanT { using type = GreedyPlan<H>; };
 template <> struct XPlanT<288> { using type = Plan<16, 18>; };
 template <> struct XPlanT<96> { using type = Plan<8, 12>; };
+template <> struct XPlanT<48> { using type = Plan<8, 6>; };
+template <> struct XPlanT<24> { using type = Plan<4, 6>; };
 template <> struct XPlanT<576> { using type = Plan<8, 8, 9>; };
 
 // Runs the passes of plan PL starting at span P (LDS to LDS).
