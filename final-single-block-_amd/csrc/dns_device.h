@@ -367,6 +367,8 @@ template <int N> using GreedyPlan = typename greedy_plan<N, Plan<>>::type;
 template <int N> struct ZPlanT { using type = GreedyPlan<N>; };
 template <> struct ZPlanT<576> { using type = Plan<12, 12, 4>; };
 template <> struct ZPlanT<1152> { using type = Plan<12, 12, 8>; };
+template <> struct ZPlanT<192> { using type = Plan<16, 12>; };
+template <> struct ZPlanT<96> { using type = Plan<8, 12>; };
 template <int H> struct XPlanT { using type = GreedyPlan<H>; };
 template <> struct XPlanT<288> { using type = Plan<16, 18>; };
 template <> struct XPlanT<96> { using type = Plan<8, 12>; };
