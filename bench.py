@@ -176,13 +176,18 @@ def large_leg(steps, warmup):
 
 def error_leg(st, nsteps):
     """Relative L2 velocity error of the mixed-precision path against FP64 after nsteps steps from the same
-    state (each path in its own handle, CFL time step; not timed)."""
+    state (each path in its own handle, CFL time step; not timed), and, for scale, the difference that an
+    FP64 run from the same state with every velocity value perturbed by a relative 1e-10 (Gaussian, seed
+    11) has grown to by then."""
     import numpy as np
     import dnship
+    rng = np.random.default_rng(11)
+    pert = [a * (1 + 1e-10 * rng.standard_normal(a.shape)) for a in st[:3]] + [st[3]]
     out = {}
-    for tag, prec in (("fp64", dnship.PREC_FP64), ("mixed", dnship.PREC_MIXED)):
+    for tag, prec, state in (("fp64", dnship.PREC_FP64, st), ("mixed", dnship.PREC_MIXED, st),
+                             ("fp64_perturbed", dnship.PREC_FP64, pert)):
         g = dnship.DNS(MX, MZ, NY, Re=RE, mpgx=MPGX, precision=prec)
-        g.upload(*st)
+        g.upload(*state)
         done = 0
         while done < nsteps:          # in chunks, so a long run reports progress
             n = min(100, nsteps - done)
@@ -192,10 +197,11 @@ def error_leg(st, nsteps):
             print("error leg: %s %d/%d steps" % (tag, done, nsteps), file=sys.stderr, flush=True)
         out[tag] = (g.download(), g.scalars())
         g.close()
-    (a, sa), (b, sb) = out["fp64"], out["mixed"]
-    return {"steps": nsteps, "t_fp64": sa["t"], "t_mixed": sb["t"],
-            "rel_l2": {k: float(np.linalg.norm(y - x) / np.linalg.norm(x))
-                       for k, x, y in zip(("u1", "u2", "u3"), a[:3], b[:3])}}
+    (a, sa), (b, sb), (c, _) = out["fp64"], out["mixed"], out["fp64_perturbed"]
+    rel = lambda u, v: {k: float(np.linalg.norm(y - x) / np.linalg.norm(x))
+                        for k, x, y in zip(("u1", "u2", "u3"), u[:3], v[:3])}
+    return {"steps": nsteps, "t_fp64": sa["t"], "t_mixed": sb["t"], "rel_l2": rel(a, b),
+            "fp64_perturbed_1e-10_rel_l2": rel(a, c)}
 
 
 def stats_leg(st, reps=10):
