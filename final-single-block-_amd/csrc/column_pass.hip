@@ -507,7 +507,7 @@ int launch_column_pass(const ColArgs &a, hipStream_t st) {
     case 128: launch_pass<8, 32, 4>(a, st); break;
     case 160: launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
     case 256: launch_pass<8, 64, 4>(a, st); break;
-    case 384: launch_pass<4, 64, 6>(a, st); break;
+    case 384: launch_pass<8, 64, 6>(a, st); break;
     case 512: launch_pass<4, 64, 8>(a, st); break;
     default: return 1;
   }
