@@ -250,8 +250,10 @@ __host__ __device__ constexpr int next_radix(int rem) {
 // that are contiguous rows).
 // TR = true maps active transform a to slot (a % GA) * GS + a / GA instead (transform-major order of the
 // active transforms, e.g. all rows of field 0 first).
+// OPQ = false lets the optimiser keep the passes' index arithmetic in registers (for kernels whose
+// occupancy LDS limits, so that registers are not the scarce resource).
 template <int N_, int NTR_, int NACT_ = NTR_, int GA_ = 1, int GS_ = 1, int PS_ = 0, bool BF_ = false,
-          bool TR_ = false>
+          bool TR_ = false, bool OPQ_ = true>
 struct Lay {
   static constexpr int N = N_, NTR = NTR_, NACT = NACT_, PS = PS_;
   static constexpr bool BF = BF_;
@@ -262,6 +264,7 @@ struct Lay {
   __device__ __forceinline__ static int slot(int a) {
     return TR_ ? (a % GA_) * GS_ + a / GA_ : (a / GA_) * GS_ + a % GA_;
   }
+  __device__ __forceinline__ static int tid();
   // thread work item b of a pass with M butterflies per transform -> (active transform a, butterfly i)
   template <int M>
   __device__ __forceinline__ static void split(int b, int &a, int &i) {
@@ -284,6 +287,11 @@ __device__ __forceinline__ int tidx() {
   int t = threadIdx.x;
   asm volatile("" : "+v"(t));
   return t;
+}
+
+template <int N_, int NTR_, int NACT_, int GA_, int GS_, int PS_, bool BF_, bool TR_, bool OPQ_>
+__device__ __forceinline__ int Lay<N_, NTR_, NACT_, GA_, GS_, PS_, BF_, TR_, OPQ_>::tid() {
+  return OPQ_ ? tidx() : (int)threadIdx.x;
 }
 
 // Butterfly of one pass on registers u[0..RAD): twiddle (P > 1), DFT.
@@ -309,7 +317,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   cx<R> u[PER][RAD];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = tidx() + q * NTH;
+    const int b = L::tid() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -321,7 +329,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = tidx() + q * NTH;
+    const int b = L::tid() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -406,7 +414,7 @@ __device__ __forceinline__ void fft_first(cx<R> *s, LD ld) {
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = tidx() + q * NTH;
+    const int b = L::tid() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -435,7 +443,7 @@ __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = tidx() + q * NTH;
+    const int b = L::tid() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<P>(b, a, i);

@@ -119,8 +119,9 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   constexpr int H = MX / 2;
   constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
   constexpr int NS = RB * NSL;
-  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true>;  // inverse: active f*RB + r -> slot r*NSL + f
-  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true>;            // forward: slots r*NSL + p, p < NP
+  // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers
+  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false>;  // inverse: active f*RB + r -> slot r*NSL + f
+  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false>;  // forward: slots r*NSL + p, p < NP
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
   const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
@@ -156,8 +157,8 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   // each thread stays on one row (NTH/RB threads per row) and strides its positions: no division
   constexpr int TPR = NTH / RB;
   static_assert(NTH % RB == 0, "threads per row");
-  cx<R> *const prow = A + (tidx() / TPR) * NSL * LI::LR;
-  for (int n = tidx() % TPR; n < H; n += TPR) {
+  cx<R> *const prow = A + ((int)threadIdx.x / TPR) * NSL * LI::LR;
+  for (int n = (int)threadIdx.x % TPR; n < H; n += TPR) {
     const int e = LI::row(n);
     cx<R> *b0 = prow + e;
     const cx<R> zu = b0[0], zv = b0[LI::LR], zw = b0[2 * LI::LR];
@@ -191,11 +192,11 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   {  // each thread stays on one (row, product) pair and strides its modes: no division per output
     constexpr int SLOTS = RB * NP, TPS = NTH / SLOTS;
     static_assert(TPS >= 1, "threads per output row");
-    const int q = tidx() / TPS, r = q / NP, p = q - r * NP;
+    const int q = (int)threadIdx.x / TPS, r = q / NP, p = q - r * NP;
     if (q < SLOTS) {
       const cx<R> *b0 = A + (r * NSL + p) * LF::LR;
       cx<R> *out = P + ((size_t)p * MZ + (z0 + r)) * Kp;
-      for (int k = tidx() % TPS; k < Kx; k += TPS) {
+      for (int k = (int)threadIdx.x % TPS; k < Kx; k += TPS) {
         const cx<R> zk = b0[LF::row(k)];
         const cx<R> zm = cconj(b0[LF::row(k == 0 ? 0 : H - k)]);
         const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
