@@ -45,18 +45,18 @@ def test_four_to_phys_program_vs_reference_golden(tmp_path, golden, case, nvar):
             ftp_compare(f.read(), g["%s_out_%s" % (case, n)].tobytes(), 1e-13)
 
 
-@pytest.mark.parametrize("M", [576, 384])
-def test_four_to_phys_full_size_vs_oracle(M):
-    """384 x 384 spectral planes to the 3/2 grid (576) and to the spectral grid (384), against the oracle's
-    transform (numpy FFTs), every entry of the input set."""
-    Nx = Nz = 384
+@pytest.mark.parametrize("Nx,Nz,Mx,Mz", [(384, 384, 576, 576), (384, 384, 384, 384), (768, 384, 1152, 576),
+                                         (768, 384, 768, 384)])
+def test_four_to_phys_full_size_vs_oracle(Nx, Nz, Mx, Mz):
+    """Spectral planes of the 384 x 384 and 768 x 384 workloads to the 3/2 grid and to the spectral grid,
+    against the oracle's transform (numpy FFTs), every entry of the input set."""
     rng = np.random.default_rng(7)
     planes = rng.standard_normal((6, Nz, Nx + 2))
-    print("case: dns_four_to_phys 384 -> %d" % M, flush=True)
-    out, ms = dnship.four_to_phys(planes, M, M)
+    print("case: dns_four_to_phys %dx%d -> %dx%d" % (Nx, Nz, Mx, Mz), flush=True)
+    out, ms = dnship.four_to_phys(planes, Mx, Mz)
     assert ms > 0
     for k in range(planes.shape[0]):
-        want = pyoracle.ftp_plane(planes[k], M, M)
+        want = pyoracle.ftp_plane(planes[k], Mx, Mz)
         assert np.abs(out[k] - want).max() <= 1e-13 * np.abs(want).max(), k
 
 
