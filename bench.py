@@ -204,6 +204,40 @@ def error_leg(st, nsteps):
             "fp64_perturbed_1e-10_rel_l2": rel(a, c)}
 
 
+# The smaller single-channel configurations: 32 x 32 x 153 (the reference's CPU-runnable case) and
+# 128 x 128 x 153 modes, Re = 3250, Re_tau = 180 (mpgx = -3.0675e-3), seeded broadband IC (seed 1).
+SMALL = [(48, 48, 153), (192, 192, 153)]
+
+
+def small_leg(steps, warmup):
+    """FP64 and mixed steps/s of single channels at the smaller configurations, each RK3 step replayed from
+    a captured hipGraph (at these sizes the launches, not HBM, set the pace without one)."""
+    import dnship
+    import dns_ic
+    import dns_roofline
+    out = {}
+    for Mx, Mz, ny in SMALL:
+        st = dns_ic.seeded_channel(Mx, Mz, ny, seed=1, amp=0.1, broadband=True)
+        key = "%dx%dx%d" % (2 * Mx // 3, 2 * Mz // 3, ny)
+        out[key] = {}
+        for tag, prec in (("fp64", dnship.PREC_FP64), ("mixed", dnship.PREC_MIXED)):
+            g = dnship.DNS(Mx, Mz, ny, Re=3250.0, mpgx=-3.0675e-3, precision=prec)
+            g.upload(*st)
+            g.autotune(iters=3)
+            g.upload(*st)
+            g.step_graph(max(warmup, 1))
+            g.sync()
+            t0 = time.perf_counter()
+            g.step_graph(steps)
+            g.sync()
+            sec = (time.perf_counter() - t0) / steps
+            g.close()
+            by = dns_roofline.reference_design_step_bytes(Mx, Mz, ny, prec == dnship.PREC_MIXED)
+            out[key][tag] = {"value": 1.0 / sec, "ms_per_step": sec * 1e3,
+                             "step_frac_hbm_reference_design": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS}
+    return out
+
+
 def stats_leg(st, reps=10):
     """dns_accumulate_stats on the 384x384x257 state (plane statistics and spectra of stats.f90 and
     spectra.f90, on the device): time per accumulation (host clock around synchronised calls) and its HBM
@@ -264,6 +298,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-post", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--error-steps", type=int, default=0,
                     help="also report the mixed path's error against FP64 after this many steps")
     args = ap.parse_args()
@@ -338,6 +373,8 @@ def main():
             out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
     if not args.no_large:
         out["large"] = large_leg(max(1, min(args.steps, 5)), 1)
+    if not args.no_small:
+        out["small"] = small_leg(max(args.steps, 20) * 5, args.warmup)
     if not args.no_post:
         out["four_to_phys"] = post_leg(f64[0])
         out["stats"] = stats_leg(st)
