@@ -89,6 +89,7 @@ def test_fused_steps_vs_reference_golden(golden, ctp, path):
     (48, 48, 33, 10, 1),
     (48, 48, 153, 10, 1),     # 32 x 32 modes x 153: the reference's CPU-runnable case
     (96, 96, 65, 5, 2),
+    (96, 96, 97, 3, 4),       # 64 x 64 x 97 (an ensemble group's size; 128-row column segments)
     (192, 192, 153, 3, 3),    # 128 x 128 x 153
     (576, 576, 17, 2, 5),     # the 384 x 384 plane transforms (tuned radix plans), few planes
     (1152, 576, 9, 1, 6),     # 768 x 384 plane transforms
