@@ -487,8 +487,8 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
 // (NC, P, M) by the rows to cover (nyu <= P*M, M >= 4).  M = 4 keeps a thread within 256 registers (two
 // wavefronts per SIMD); longer segments cost one wavefront per SIMD.
 int column_pass_rows(int nyu) {
-  return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 128 ? 128 : nyu <= 160 ? 160 : nyu <= 256 ? 256
-         : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
+  return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 96 ? 96 : nyu <= 128 ? 128 : nyu <= 160 ? 160
+         : nyu <= 192 ? 192 : nyu <= 256 ? 256 : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
 }
 
 bool column_pass_supported(int nyu) { return nyu >= 8 && column_pass_rows(nyu) > 0; }
@@ -504,8 +504,10 @@ int launch_column_pass(const ColArgs &a, hipStream_t st) {
   switch (column_pass_rows(a.nyu)) {
     case 32: launch_pass<8, 8, 4>(a, st); break;
     case 64: launch_pass<8, 16, 4>(a, st); break;
+    case 96: launch_pass<8, 32, 3>(a, st); break;
     case 128: launch_pass<8, 32, 4>(a, st); break;
     case 160: launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
+    case 192: launch_pass<8, 64, 3>(a, st); break;
     case 256: launch_pass<8, 64, 4>(a, st); break;
     case 384: launch_pass<8, 64, 6>(a, st); break;
     case 512: launch_pass<4, 64, 8>(a, st); break;
