@@ -484,8 +484,10 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
   }
 }
 
-// (NC, P, M) by the rows to cover (nyu <= P*M, M >= 4).  M = 4 keeps a thread within 256 registers (two
-// wavefronts per SIMD); longer segments cost one wavefront per SIMD.
+// (NC, P, M) by the rows to cover (nyu <= P*M, M >= 3: a segment needs a first, an interior and a last
+// row).  M = 4 keeps a thread within 256 registers (two wavefronts per SIMD); longer segments spill and
+// run one wavefront per SIMD.  Each bucket's shape was timed against alternatives (3-row segments at 96
+// rows, 16 columns x 32 x 5 at the 153-point grids, 8 columns x 64 x 6 at 385 points).
 int column_pass_rows(int nyu) {
   return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 96 ? 96 : nyu <= 128 ? 128 : nyu <= 160 ? 160
          : nyu <= 192 ? 192 : nyu <= 256 ? 256 : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
