@@ -95,6 +95,8 @@ def test_fused_steps_vs_reference_golden(golden, ctp, path):
     (1152, 576, 9, 1, 6),     # 768 x 384 plane transforms
     (24, 24, 257, 4, 7),      # the 257-point y grid of the 384 x 384 x 257 workload
     (24, 24, 385, 2, 8),      # 385 points (one-pass column stage with 6-row segments)
+    (24, 24, 193, 2, 10),     # 64 segments of 3 rows
+    (24, 24, 513, 2, 11),     # 64 segments of 8 rows
 ])
 @pytest.mark.parametrize("path", [0, 1])
 def test_fused_steps_vs_oracle(Mx, Mz, ny, nsteps, seed, path):
