@@ -231,6 +231,12 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
   constexpr int NTH = NC * P;
   __shared__ PassShared<NC, P> sh;
   __shared__ double u1q[P * M];  // column 0's u1 for the flow rate
+  // The segment's p rows from the first phase, kept in LDS for p += psi at the end instead of being read
+  // from HBM again: with 64 segments (one workgroup per CU) where LDS has room; with fewer segments the
+  // stash made the compiler spill.
+  constexpr bool PST = P >= 64 && sizeof(PassShared<NC, P>) + sizeof(double) * P * M +
+                                       sizeof(double2) * M * NC * P <= 160 * 1024;
+  __shared__ double2 pst[PST ? M * NC * P : 1];
   const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
   const NA<NT> Nu1{a.Nu1.p}, Nu2{a.Nu2.p}, Nu3{a.Nu3.p}, Nxz1{a.Nxz1.p}, Nxz2{a.Nxz2.p}, Nxz3{a.Nxz3.p};
   const NA<NT> uvf{a.uvf.p}, vvc{a.vvc.p}, wvf{a.wvf.p};
@@ -281,6 +287,7 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
       const int j = j0 + r;
       const bool ur = j <= nyu, vr = j <= nyv;
       const int ju = cp(j), jv = cq(j);
+      if constexpr (PST) pst[r * NTH + tid] = pc;  // p row cp(j)
       // u-grid row j
       const double2 u1p = a.u1[at(cu(j + 1))], u3p = a.u3[at(cu(j + 1))];
       const double2 uvc = uvf[at(cv(j))], wvc = wvf[at(cv(j))];
@@ -422,7 +429,10 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
     auto at = [&](int j) { return (size_t)j * C + cl; };
     double2 pold[M];
 #pragma unroll
-    for (int r = 0; r < M; r++) pold[r] = a.p[at(cp(jb + r) - 1)];
+    for (int r = 0; r < M; r++) {
+      if constexpr (PST) pold[r] = pst[r * NTH + tid];
+      else pold[r] = a.p[at(cp(jb + r) - 1)];
+    }
 #pragma unroll
     for (int r = 0; r < M; r++) {
       const int j = jb + r;
