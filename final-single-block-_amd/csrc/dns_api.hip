@@ -269,7 +269,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   char *Ib = nullptr, *Pb = nullptr, *tz = nullptr, *tx = nullptr, *th = nullptr;
   rc |= chk(dalloc(h, &Ib, iel * rsz)); rc |= chk(dalloc(h, &Pb, pel * rsz));
   rc |= chk(dalloc(h, &tz, (size_t)Mz * rsz)); rc |= chk(dalloc(h, &tx, (size_t)Mx * rsz));
-  rc |= chk(dalloc(h, &th, (size_t)(Mx / 2) * rsz));
+  rc |= chk(dalloc(h, &th, (size_t)2 * (Mx / 2) * rsz));  // the table, then its pass-blocked copy
   rc |= chk(dalloc(h, &h->twz64, (size_t)Mz)); rc |= chk(dalloc(h, &h->twx64, (size_t)Mx));
   rc |= chk(dalloc(h, &h->twh64, (size_t)Mx / 2));
   if (rc) {
@@ -297,6 +297,12 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
     e = e ? e : up(tx, b.data(), b.size() * 8);
     e = e ? e : up(th, c.data(), c.size() * 8);
   }
+  if (!e && dnsk::build_xpass_twiddles(th, Mx, h->fp32, nullptr) != 0) {
+    destroy_all(h);
+    return fail(DNS_ERR_ARG, "dns_create: no x-pass plan for Mx = %d", Mx);
+  }
+  e = e ? e : hipDeviceSynchronize();
+  e = e ? e : hipGetLastError();
   {
     auto a = twiddles<double>(Mz), b = twiddles<double>(Mx), c = twiddles<double>(Mx / 2);
     e = e ? e : up(h->twz64, a.data(), a.size() * 8);

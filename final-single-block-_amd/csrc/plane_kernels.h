@@ -119,9 +119,10 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   constexpr int H = MX / 2;
   constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
   constexpr int NS = RB * NSL;
-  // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers
-  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false>;  // inverse: active f*RB + r -> slot r*NSL + f
-  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false>;  // forward: slots r*NSL + p, p < NP
+  // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers; twiddles from
+  // the pass-blocked copy after twh (contiguous loads across the butterflies of a pass)
+  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false, true>;  // inverse: active f*RB + r -> slot r*NSL + f
+  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false, true>;  // forward: slots r*NSL + p, p < NP
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
   const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   constexpr int Kx = MX / 3 + 1;   // retained x-modes Nx/2 + 1, Nx = 2 MX/3 (the launcher checks a.Kx)
   const int MZ = a.Mz;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
-  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}
+  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}, m < H, then its pass blocks
   constexpr int Kp = row_pitch<R>(Kx);
   const size_t fsz = (size_t)MZ * Kp;
   const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);

@@ -278,6 +278,24 @@ static int launch_t(const PlaneArgs &a, int nw, int which, int v, hipStream_t st
   return -1;
 }
 
+// The x-pass's twiddle table twh holds H = Mx/2 entries e^{-2 pi i m/H} followed by H entries of their
+// pass-blocked copy for the x-pass plan (pass_twiddles_kernel); this fills the copy.
+int build_xpass_twiddles(void *twh, int Mx, bool fp32, hipStream_t st) {
+#define CASE(N)                                                                                          \
+  if (Mx == N) {                                                                                         \
+    if (fp32)                                                                                            \
+      pass_twiddles_kernel<float, N / 2, typename XPlanT<N / 2>::type><<<(N / 2 + 255) / 256, 256, 0, st>>>( \
+          static_cast<cx<float> *>(twh));                                                                \
+    else                                                                                                 \
+      pass_twiddles_kernel<double, N / 2, typename XPlanT<N / 2>::type><<<(N / 2 + 255) / 256, 256, 0, st>>>( \
+          static_cast<cx<double> *>(twh));                                                               \
+    return 0;                                                                                            \
+  }
+  DNS_SIZES(CASE)
+#undef CASE
+  return -1;
+}
+
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st) {
   if (a.Kx != a.Mx / 3 + 1 || a.Nz != 2 * a.Mz / 3) return -1;  // the kernels take both from Mx, Mz
   return fp32 ? launch_t<float>(a, nw, which, variant, st) : launch_t<double>(a, nw, which, variant, st);
