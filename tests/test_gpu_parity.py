@@ -97,6 +97,10 @@ def test_fused_steps_vs_reference_golden(golden, ctp, path):
     (24, 24, 385, 2, 8),      # 385 points (one-pass column stage with 6-row segments)
     (24, 24, 193, 2, 10),     # 64 segments of 3 rows
     (24, 24, 513, 2, 11),     # 64 segments of 8 rows
+    (12, 12, 17, 3, 12),      # the smallest plane, 8 x 8 modes
+    (144, 288, 9, 1, 13),     # Mx != Mz: x half-length 72 (plan 9.8), z 288
+    (288, 144, 9, 1, 14),     # x half-length 144, z 144
+    (384, 384, 9, 1, 15),     # 256 x 256 modes: x half-length 192, z 384
 ])
 @pytest.mark.parametrize("path", [0, 1])
 def test_fused_steps_vs_oracle(Mx, Mz, ny, nsteps, seed, path):
