@@ -123,6 +123,9 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   // the pass-blocked copy after twh (contiguous loads across the butterflies of a pass)
   using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false, true>;  // inverse: active f*RB + r -> slot r*NSL + f
   using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false, true>;  // forward: slots r*NSL + p, p < NP
+  static_assert(std::is_same<PLI, typename XPlanT<H>::type>::value &&
+                    std::is_same<PLF, typename XPlanT<H>::type>::value,
+                "twh's pass-blocked twiddles are built for the plan XPlanT<H>");
   __shared__ cx<R> A[LI::SIZE];
   __shared__ double red[NTH / 64];
   const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
