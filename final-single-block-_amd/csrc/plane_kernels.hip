@@ -159,7 +159,7 @@ constexpr size_t LDS_MAX = 160 * 1024;
 // of each field); 12.12.x plans: 48 threads per column (one butterfly each), at most 768 threads.
 // 576: variant 1 is variant 0 with 4 planes per z-inverse workgroup (fewer rows read twice); variant 2
 // takes 32 columns (two butterflies per thread; 512-byte FP32 row segments, an LDS tile that only FP32
-// fits, FP64 falling back to variant 0).
+// fits); in FP64 it is variant 0 with 8 planes per z-inverse workgroup instead (tile 3).
 // Other lengths: 16, 8 or 32 columns, with threads for about one butterfly of the 16-point first pass
 // each (64 to 256).
 template <int N, int V> struct ZTile {
@@ -168,7 +168,7 @@ template <int N, int V> struct ZTile {
                          : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 16 : V == 1 ? 8 : 32);
   static constexpr int NB = (T * N / 16 + 63) / 64 * 64;  // first-pass butterflies, whole wavefronts
   static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (NB < 64 ? 64 : NB > 256 ? 256 : NB);
-  static constexpr int ZCH = (N == 576 && V == 1) ? 4 : 2;
+  static constexpr int ZCH = (N == 576 && V == 1) ? 4 : (N == 576 && V == 3) ? 8 : 2;
 };
 // x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront per
 // row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only four
@@ -199,7 +199,7 @@ template <typename R, int MZ, int V>
 static void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
   using Z = ZTile<MZ, V>;
   if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
-    zinv_v<R, MZ, 0>(a, nw, st);
+    zinv_v<R, MZ, (MZ == 576 && V == 2) ? 3 : 0>(a, nw, st);
   } else {
     (void)nw;
     dim3 g((a.Kx + Z::T - 1) / Z::T, 3, ((a.nbj + Z::ZCH - 1) / Z::ZCH) * a.nm);
