@@ -195,6 +195,7 @@ static std::vector<R> twiddles(int n) {  // e^{-2 pi i m/n} as (re, im) pairs
 }
 
 static void destroy_all(dns_handle *h) {
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->graph) (void)hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->stream) {
@@ -793,16 +794,21 @@ extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
   if (!h->graph) {
     const bool tm = h->timing;
     h->timing = false;
-    hipGraph_t g;
+    hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int kRK = 1; kRK <= 3 && rc == 0; kRK++) rc = fused_substep(h, kRK);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->timing = tm;
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(DNS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    if (rc || e != hipSuccess) {  // the captured graph (if any) is not kept
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      if (rc) return rc;
+      return fail(DNS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(DNS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    h->graph = x;
   }
   for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(h->graph, h->stream));
   return DNS_OK;
@@ -818,8 +824,13 @@ static int kern_variants(const dns_handle *h, int c) {
   return 3;
 }
 
+// Replays of the executable graph may still be queued on the handle's stream: wait for them before the
+// executable is destroyed.
 static void drop_graph(dns_handle *h) {
-  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  if (h->graph) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipGraphExecDestroy(h->graph);
+  }
   h->graph = nullptr;
 }
 

@@ -1,6 +1,6 @@
 // Plane stage of the nonlinear term: CFL helpers, the reference-format plane round trip, and the
 // launchers of the plane kernels (plane_kernels.h) for every supported transform length.
-#include "plane_kernels.h"
+#include "plane_launch.h"
 
 namespace dnsk {
 
@@ -142,48 +142,6 @@ __global__ void __launch_bounds__(NTH) plane_roundtrip_kernel(const double *in, 
   for (int e = threadIdx.x; e < IG; e += NTH) pbk[(MZ / 2) * IG + e] = 0.0;
 }
 
-// ------------------------------------------------------------------------------------------------
-// launchers.  The radix plan is fixed per transform length (so every tile variant of a kernel gives the
-// same bits); the tile variants differ in columns/rows per workgroup and threads, and the autotuner
-// picks among them.  Plans and tiles of the tuned lengths were chosen by timing on MI355X:
-//   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column),
-//     16 columns per workgroup (256-byte row segments in FP64);
-//   x-pass half-length 288: 16.18 (two passes).
-#define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
-#define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
-
-
-constexpr size_t LDS_MAX = 160 * 1024;
-
-// z-kernel tiles (T columns, NTH threads, ZCH planes per z-inverse workgroup, which reads ZCH + 1 rows
-// of each field); 12.12.x plans: 48 threads per column (one butterfly each), at most 768 threads.
-// 576: variant 1 is variant 0 with 4 planes per z-inverse workgroup (fewer rows read twice); variant 2
-// takes 32 columns (two butterflies per thread; 512-byte FP32 row segments, an LDS tile that only FP32
-// fits); in FP64 it is variant 0 with 8 planes per z-inverse workgroup instead (tile 3).
-// Other lengths: 16, 8 or 32 columns, with threads for about one butterfly of the 16-point first pass
-// each (64 to 256).
-template <int N, int V> struct ZTile {
-  static constexpr bool p12 = (N == 576 || N == 1152);
-  static constexpr int T = N == 576 ? (V == 2 ? 32 : 16)
-                         : N == 1152 ? (V == 0 ? 8 : V == 1 ? 4 : 16) : (V == 0 ? 16 : V == 1 ? 8 : 32);
-  static constexpr int NB = (T * N / 16 + 63) / 64 * 64;  // first-pass butterflies, whole wavefronts
-  static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (NB < 64 ? 64 : NB > 256 ? 256 : NB);
-  static constexpr int ZCH = (N == 576 && V == 1) ? 4 : (N == 576 && V == 3) ? 8 : 2;
-};
-// x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront per
-// row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only four
-// rows fit a CU): one row per workgroup with 2, 4 or 1 wavefronts.  Shorter rows have too few butterflies
-// per pass for a wavefront, so RPW rows share one: workgroups of 2, 1 or 4 wavefronts.
-template <int MX, int V> struct XTile {
-  static constexpr int H = MX / 2;
-  static constexpr int RPW = H <= 48 ? 4 : H <= 96 ? 2 : 1;   // rows per wavefront below 576 points
-  static constexpr bool big = MX >= 1152, mid = MX == 576;
-  static constexpr int RB = big ? 1 : mid ? (V == 0 ? 4 : V == 1 ? 2 : 1)
-                                         : (V == 0 ? 2 * RPW : V == 1 ? RPW : 4 * RPW);
-  static constexpr int NTH = big ? (V == 0 ? 128 : V == 1 ? 256 : 64) : mid ? (V == 0 ? 256 : V == 1 ? 128 : 64)
-                                                                          : (V == 0 ? 128 : V == 1 ? 64 : 256);
-};
-
 bool plane_size_supported(int n) {
 #define CASE(N) if (n == N) return true;
   DNS_SIZES(CASE)
@@ -195,86 +153,15 @@ int xpass_rows_per_block(int Mx) { (void)Mx; return 1; }  // every Mz: tiles tha
 
 int plane_variants(int n) { return DNS_TUNED(n) ? 3 : 1; }
 
-template <typename R, int MZ, int V>
-static void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
-  using Z = ZTile<MZ, V>;
-  if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
-    zinv_v<R, MZ, (MZ == 576 && V == 2) ? 3 : 0>(a, nw, st);
-  } else {
-    (void)nw;
-    dim3 g((a.Kx + Z::T - 1) / Z::T, 3, ((a.nbj + Z::ZCH - 1) / Z::ZCH) * a.nm);
-    zinv_kernel<R, MZ, Z::T, Z::NTH, typename ZPlanT<MZ>::type, Z::ZCH, 1, 0><<<g, Z::NTH, 0, st>>>(a);
-  }
-}
-template <typename R, int MZ, int V, bool VPL>
-static void zfwd_v(const PlaneArgs &a, int nw, hipStream_t st) {
-  using Z = ZTile<MZ, V>;
-  if constexpr (V > 0 && (size_t)Z::T * MZ * sizeof(cx<R>) > LDS_MAX) {
-    zfwd_v<R, MZ, 0, VPL>(a, nw, st);
-  } else {
-    dim3 g((a.Kx + Z::T - 1) / Z::T, nw * a.nm);
-    zfwd_kernel<R, MZ, Z::T, Z::NTH, VPL, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
-  }
-}
-template <typename R, int MX, int V, bool VPL>
-static void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
-  using X = XTile<MX, V>;
-  constexpr int H = MX / 2;
-  if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
-    xpass_v<R, MX, 0, VPL>(a, nw, st);
-  } else {
-    using PL = typename XPlanT<H>::type;
-    if (a.Mz % X::RB != 0) {  // rows that do not fill the tile: one row per single-wavefront workgroup
-      xpass_kernel<R, MX, 1, 64, VPL, PL, PL, 1, 4><<<dim3(a.Mz, nw * a.nm), 64, 0, st>>>(a);
-      return;
-    }
-    dim3 g(a.Mz / X::RB, nw * a.nm);
-    xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);
-  }
-}
-
-template <typename R, int N>
-static void zinv_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
-  if constexpr (DNS_TUNED(N)) {
-    if (v == 1) return zinv_v<R, N, 1>(a, nw, st);
-    if (v == 2) return zinv_v<R, N, 2>(a, nw, st);
-  }
-  zinv_v<R, N, 0>(a, nw, st);
-}
-template <typename R, int N, bool VPL>
-static void zfwd_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
-  if constexpr (DNS_TUNED(N)) {
-    if (v == 1) return zfwd_v<R, N, 1, VPL>(a, nw, st);
-    if (v == 2) return zfwd_v<R, N, 2, VPL>(a, nw, st);
-  }
-  zfwd_v<R, N, 0, VPL>(a, nw, st);
-}
-template <typename R, int N, bool VPL>
-static void xpass_sized(const PlaneArgs &a, int nw, int v, hipStream_t st) {
-  if constexpr (DNS_TUNED(N)) {
-    if (v == 1) return xpass_v<R, N, 1, VPL>(a, nw, st);
-    if (v == 2) return xpass_v<R, N, 2, VPL>(a, nw, st);
-  }
-  xpass_v<R, N, 0, VPL>(a, nw, st);
-}
-
 // which: 0 z-inverse, 1 x-pass u-planes, 2 x-pass v-planes, 3 z-forward u-planes, 4 z-forward v-planes.
 // The batch is a.j0, a.nbj; nw = blockIdx.y extent (a.nbj + 1 field rows for the z-inverse, else a.nbj).
-template <typename R>
-static int launch_t(const PlaneArgs &a, int nw, int which, int v, hipStream_t st) {
-  const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
-#define CASE(N)                                                       \
-  if (n == N) {                                                       \
-    switch (which) {                                                  \
-      case 0: zinv_sized<R, N>(a, nw, v, st); return 0;               \
-      case 1: xpass_sized<R, N, false>(a, nw, v, st); return 0;       \
-      case 2: xpass_sized<R, N, true>(a, nw, v, st); return 0;        \
-      case 3: zfwd_sized<R, N, false>(a, nw, v, st); return 0;        \
-      case 4: zfwd_sized<R, N, true>(a, nw, v, st); return 0;         \
-    }                                                                 \
+static int launch_which(const PlaneArgs &a, int nw, bool fp32, int which, int v, hipStream_t st) {
+  switch (which) {
+    case 0: return launch_zinv(a, nw, fp32, v, st);
+    case 1: case 2:
+      return fp32 ? launch_xpass_f32(a, nw, which == 2, v, st) : launch_xpass_f64(a, nw, which == 2, v, st);
+    case 3: case 4: return launch_zfwd(a, nw, fp32, which == 4, v, st);
   }
-  DNS_SIZES(CASE)
-#undef CASE
   return -1;
 }
 
@@ -298,7 +185,7 @@ int build_xpass_twiddles(void *twh, int Mx, bool fp32, hipStream_t st) {
 
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st) {
   if (a.Kx != a.Mx / 3 + 1 || a.Nz != 2 * a.Mz / 3) return -1;  // the kernels take both from Mx, Mz
-  return fp32 ? launch_t<float>(a, nw, which, variant, st) : launch_t<double>(a, nw, which, variant, st);
+  return launch_which(a, nw, fp32, which, variant, st);
 }
 
 void launch_cfl_ghost(const double2 *u1, const double2 *u3, int C, size_t ms, int nm, double *cfl, hipStream_t st) {

@@ -329,7 +329,7 @@ class DNS:
         _chk(lib().dns_get_column_path(self.h, ctypes.byref(v)))
         return v.value
 
-    # ---- device-pointer utilities (torch tensors provide the device memory)
+    # ---- device-pointer utilities (the caller owns the device memory: any HIP allocation)
     def plane_fft_roundtrip(self, d_in, d_phys, d_back, n):
         _chk(lib().dns_plane_fft_roundtrip(self.h, d_in, d_phys, d_back, n))
 
