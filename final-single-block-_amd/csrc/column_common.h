@@ -30,22 +30,13 @@ __device__ __forceinline__ Mode col_mode(const ColArgs &a, int col) {
 
 // dt for this substep; at kRK = 1 the CFL rule of FOU3D.f90:290-303 with dtc_calc's maxima.
 // dthetavi is never assigned in the reference (declaration.f90:64), so its v term is 1/(u2m*0) = Inf.
+// With a fixed dt (a.fixed) every step takes a.fixed_dt and dtc stays the CFL diagnostic.
 __device__ __forceinline__ double substep_dt(const ColArgs &a, int col) {
   if (a.kRK != 1) return a.st->dt;
-  double dt;
-  if (a.fixed) {
-    dt = a.fixed_dt;
-    if (col == 0) {
-      a.st->dt = dt;
-      a.st->dti = 1.0 / dt;
-      a.st->t = a.st->t + dt;
-    }
-    return dt;
-  }
   const double u1m = a.st->cfl[0], u3m = a.st->cfl[1];
   const double d1 = 1.0 / (a.alp * (double)(a.Nx / 2) * u1m), d3 = 1.0 / (a.bet * (double)(a.Nz / 2) * u3m);
   const double dtc = fmin(d1, d3);
-  dt = fmin(2.5 * a.dtv, a.CFL * dtc);
+  const double dt = a.fixed ? a.fixed_dt : fmin(2.5 * a.dtv, a.CFL * dtc);
   if (col == 0) {
     a.st->dtc = dtc;
     a.st->dt = dt;

@@ -309,10 +309,13 @@ contains
     call read_one('p', p, 1, nyu, tt, it)
   end subroutine
 
-  ! record_out's file layout (FOU3D.f90:2063-2161) for one field
-  subroutine write_one(name, x, j0, j1, y, dthdy, gridid)
+  ! record_out's file layout (FOU3D.f90:2063-2161) for one field: planes j0..jw, those past the field's
+  ! rows (j > j1) all zero.  One rank writes the p planes limPL_incw(pgrid) = 1..nyu+1 (start.f90:1187-1194,
+  ! FOU3D.f90:2150-2153); modes_to_planes_UVP fills ppPL planes 1..nyu only, and ppPL was zeroed before it
+  ! (FOU3D.f90:151-152, littleharsh.f90:439-443), so plane nyu+1 is a zero plane.
+  subroutine write_one(name, x, j0, j1, jw, y, dthdy, gridid)
     character(*), intent(in) :: name
-    integer, intent(in) :: j0, j1, gridid
+    integer, intent(in) :: j0, j1, jw, gridid
     complex(8), intent(in) :: x(j0:j1, ncol)
     real(8), intent(in) :: y(0:), dthdy(0:)
     integer :: N(4, 0:4), dummint(88), j, k, i, col
@@ -337,14 +340,17 @@ contains
     write(10) t, Re, alp, bet, mpgx, 3, iter, dummint
     write(10) N
     write(10) y, 1d0, dthdy
-    do j = j0, j1
-      do k = 1, Nspec_z
-        do i = 0, Nspec_x/2
-          col = (k-1)*(Nspec_x/2+1) + i + 1
-          buff(2*i+1, k) = dble(x(j, col))
-          buff(2*i+2, k) = dimag(x(j, col))
+    do j = j0, jw
+      buff = 0d0
+      if (j <= j1) then
+        do k = 1, Nspec_z
+          do i = 0, Nspec_x/2
+            col = (k-1)*(Nspec_x/2+1) + i + 1
+            buff(2*i+1, k) = dble(x(j, col))
+            buff(2*i+2, k) = dimag(x(j, col))
+          end do
         end do
-      end do
+      end if
       write(10) j, gridid, Nspec_x+2, Nspec_z, y(j), buff
     end do
     close(10)
@@ -515,10 +521,10 @@ contains
     real(8), allocatable :: yu(:), yv(:), du(:), dv(:)
     allocate(yu(0:ny), yv(0:ny-1), du(0:ny), dv(0:ny-1))
     call dns_hip_check(dns_ygrid(int(ny, c_int), dyq, int(ppp, c_int), yu, yv, du, dv), 'ygrid')
-    call write_one('u1', u1, 0, nyu+1, yu, du, 1)
-    call write_one('u2', u2, 0, nyv+1, yv, dv, 2)
-    call write_one('u3', u3, 0, nyu+1, yu, du, 3)
-    call write_one('p', p, 1, nyu, yu, du, 4)
+    call write_one('u1', u1, 0, nyu+1, nyu+1, yu, du, 1)
+    call write_one('u2', u2, 0, nyv+1, nyv+1, yv, dv, 2)
+    call write_one('u3', u3, 0, nyu+1, nyu+1, yu, du, 3)
+    call write_one('p', p, 1, nyu, nyu+1, yu, du, 4)
   end subroutine
 
 end program

@@ -183,6 +183,7 @@ struct orc {
   int Mx, Mz, Nx, Nz, Kx, C, ny, nyu, nyv, nthreads, ppp, flag_ctpress, iter;
   int nu_rows, nv_rows, np_rows; /* ny+1, ny, ny-1 */
   double Re, alp, bet, CFL, mpgx, dyq, t, dt, dti, dtc, dtv, Qx, QxT, dgx;
+  double fixed_dt;  /* > 0: every step takes this dt instead of the CFL rule */
   double gw[2], gbc_u1, gbc_u3, gi[2];
   double aRK[3], bRK[3], gRK[3], cRK[3], dRK[3];
   double *yu, *yv, *dthdyu, *dthdyv, *dyu2i, *dyv2i; /* dy?2i[3*j + (c-1)] */
@@ -490,6 +491,8 @@ void orc_get_scalars(const orc_t *o, double *sc) {
   sc[4] = o->Qx; sc[5] = o->dgx; sc[6] = o->dtc; sc[7] = o->QxT;
 }
 
+void orc_set_fixed_dt(orc_t *o, double dt) { o->fixed_dt = dt; }
+
 void orc_set_dt(orc_t *o, double dt) {
   o->dt = dt;
   o->dti = 1.0 / dt;
@@ -691,7 +694,8 @@ void orc_nonlinear(orc_t *o, int kRK) {
     const double d1 = 1.0 / (o->alp * (o->Nx / 2) * u1m), d3 = 1.0 / (o->bet * (o->Nz / 2) * u3m);
     const double dloc = fmin(d1, d3);
     o->dtc = dloc;
-    o->dt = fmin(2.5 * o->dtv, o->CFL * o->dtc);
+    /* fixed_dt > 0: the time step is given (a run at fixed dt; dtc stays the CFL diagnostic) */
+    o->dt = o->fixed_dt > 0.0 ? o->fixed_dt : fmin(2.5 * o->dtv, o->CFL * o->dtc);
     o->dti = 1.0 / o->dt;
     o->t = o->t + o->dt;
   }

@@ -39,6 +39,9 @@ void orc_get_psi(const orc_t *o, double *psi, double *div);
 /* sc[0..7] = t, dt, dti, mpgx, Qx, dgx, dtc, QxT */
 void orc_get_scalars(const orc_t *o, double *sc);
 void orc_set_dt(orc_t *o, double dt);
+/* dt > 0: every RK3 step takes this dt at kRK = 1 instead of the CFL rule of FOU3D.f90:290-303 (a run at
+ * fixed dt); t advances by dt, dtc is still the CFL diagnostic.  0 restores the CFL rule. */
+void orc_set_fixed_dt(orc_t *o, double dt);
 
 /* Set-up arrays: yu(ny+1) yv(ny) dthdyu(ny+1) dthdyv(ny) dyu2i(3*(ny+1)) dyv2i(3*ny), DG(3*(ny-1)*C),
  * w[0..5] = dtv, gw1, gw2, gbc, gi1, gi2 */

@@ -62,6 +62,7 @@ def lib():
         L.orc_get_psi.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_get_scalars.argtypes = [ctypes.c_void_p, _dp]
         L.orc_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_fixed_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_get_setup.argtypes = [ctypes.c_void_p] + [_dp] * 8
         L.orc_plane_fft.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         _lib = L
@@ -70,11 +71,13 @@ def lib():
 
 class Oracle:
     def __init__(self, Mx, Mz, ny, Re=3250.0, alp=1.0, bet=2.0, CFL=0.7, mpgx=-3.0675e-3, dyq=10.0,
-                 ppp=5, flag_ctpress=1, threads=None):
+                 ppp=5, flag_ctpress=1, threads=None, fixed_dt=0.0):
         L = lib()
         self.h = L.orc_create(Mx, Mz, ny, Re, alp, bet, CFL, mpgx, dyq, ppp, flag_ctpress)
         if not self.h:
             raise ValueError("unsupported sizes %d x %d" % (Mx, Mz))
+        if fixed_dt > 0:
+            L.orc_set_fixed_dt(self.h, fixed_dt)
         s = (ctypes.c_int * 8)()
         L.orc_sizes(self.h, s)
         self.Nx, self.Nz, self.Kx, self.C, self.nyu, self.nyv = s[0], s[1], s[2], s[3], s[4], s[5]

@@ -148,13 +148,22 @@ program ref_harness
   end if
 
   if (mode == 1) then
+    ! dt_in > 0: a run at fixed dt.  The reference has only the CFL rule (FOU3D.f90:290-303), which sets
+    ! dt, dti and t inside nonlinear at kRK = 1; the harness replaces them right after that call, before
+    ! anything reads them (solveU is the first user of dt).
     open(43, file='hist.txt', form='formatted')
     do istep = 1, nsteps
       do kRK = 1, 3
         call RHS0_u1(du1, u1, Nu1, p, 0)
         call RHS0_u2(du2, u2, Nu2, p, 0)
         call RHS0_u3(du3, u3, Nu3, p, 0)
+        t_begin = t
         call nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, 0, status, ierr)
+        if (kRK == 1 .and. dt_in > 0d0) then
+          dt = dt_in
+          dti = 1d0/dt
+          t = t_begin + dt
+        end if
         call solveU(u1, du1, ugrid, a_ugrid, 0)
         call solveU(u2, du2, vgrid, a_vgrid, 0)
         call solveU(u3, du3, ugrid, a_ugrid, 0)

@@ -23,7 +23,15 @@ Fixtures (inputs and expected outputs only):
                               with record_out's current header; case B: four_to_phys_N on files with
                               a three-band header like the reference's own input files.
 
-``python make_golden.py stats`` (or ``ftp``) writes only that one.
+  golden_traj_128x128x153.npz: norms and 10 columns of the reference run from the 128x128x153 seeded IC
+                              after 10, 100 and 1000 steps, the per-step (t, dt) history, and the same
+                              from a 1e-15-perturbed IC (the flow's own amplification, for scale).
+  golden_fixed_dt_32x32x153.npz: 10 steps at fixed dt = 1e-3 from the 32x32x153 seeded IC: every column's
+                              and every row's norm, 16 whole columns, the per-step history.
+  golden_dg.npz             : the reference's pressure matrices DG (LU_buildP/LU_decP).
+  fixture_32x32x153.npz     : the reference's own 32x32x153 initial fields (input/*.dat), as data.
+
+``python make_golden.py NAME...`` writes only those (stats, ftp, traj128, fixed_dt, dg, fixture32).
 """
 import subprocess
 import tempfile
@@ -127,17 +135,99 @@ def ftp_golden():
     np.savez_compressed(os.path.join(HERE, "golden_ftp_16x16x9.npz"), **out)
 
 
+TRAJ_COLS = np.array([0, 1, 2, 64, 65, 130, 1000, 4159, 4160, 8319])
+TRAJ_STOPS = (10, 100, 1000)
+
+
+def traj128_run(pert: bool):
+    """The reference's own code from the 128x128x153 seeded IC (seed 1, amplitude 0.1, lowest 8 modes;
+    pert: every velocity value times 1 + 1e-15 N(0,1), rng seed 1), CFL steps, chained runs stopping at
+    TRAJ_STOPS: per-field norms and the TRAJ_COLS columns at each stop, and the per-step (t, dt) history."""
+    Mx, Mz, ny = 192, 192, 153
+    st = list(dns_ic.seeded_channel(Mx, Mz, ny, seed=1))
+    if pert:
+        rng = np.random.default_rng(1)
+        st = [a * (1 + 1e-15 * rng.standard_normal(a.shape)) for a in st[:3]] + [st[3]]
+    out = {}
+    t, it, done, hist = 0.0, 0, 0, []
+    for stop in TRAJ_STOPS:
+        r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=stop - done, t=t, it=it)
+        st = [r[k] for k in ("u1", "u2", "u3", "p")]
+        t, it, done = r["t"], it + stop - done, stop
+        hist.append(r["hist"])
+        for k, a in zip(("u1", "u2", "u3", "p"), st):
+            out["norm_%s_%d" % (k, stop)] = np.linalg.norm(a)
+            out["cols_%s_%d" % (k, stop)] = a[TRAJ_COLS]
+        print("traj128 %s: %d steps" % ("perturbed" if pert else "plain", stop), flush=True)
+    out["hist"] = np.concatenate(hist)
+    return out
+
+
+def traj128_golden():
+    """golden_traj_128x128x153.npz: the reference's trajectory from the seeded IC over 1000 steps, and for
+    scale the trajectory from the 1e-15-perturbed IC (keys prefixed pert_); both runs in parallel."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(2) as ex:
+        a, b = ex.map(traj128_run, (False, True))
+    out = {"cols": TRAJ_COLS, "stops": np.array(TRAJ_STOPS)}
+    out.update(a)
+    out.update({"pert_" + k: v for k, v in b.items()})
+    st = dns_ic.seeded_channel(192, 192, 153, seed=1)
+    out.update({"ic_norm_" + k: np.linalg.norm(x) for k, x in zip(("u1", "u2", "u3", "p"), st)})
+    np.savez_compressed(os.path.join(HERE, "golden_traj_128x128x153.npz"), **out)
+
+
+def fixed_dt_golden():
+    """golden_fixed_dt_32x32x153.npz: 32x32x153 (48x48 dealiased), Re 3250, the seeded IC (seed 1,
+    amplitude 0.1, lowest 8 modes), 10 RK3 steps at the fixed dt = 1e-3 (the harness sets dt, dti and t after
+    the reference's nonlinear at kRK = 1)."""
+    Mx, Mz, ny = 48, 48, 153
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=1)
+    r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=10, dt=1e-3)
+    cols = np.array([0, 1, 2, 8, 16, 17, 18, 33, 100, 136, 271, 272, 288, 400, 527, 543])
+    out = {"hist": r["hist"], "cols": cols}
+    out.update({"ic_norm_" + k: np.linalg.norm(x) for k, x in zip(("u1", "u2", "u3", "p"), st)})
+    for k in ("u1", "u2", "u3", "p"):      # every column's and every row's norm, and 16 whole columns
+        out["colnorm_" + k] = np.linalg.norm(r[k], axis=1)
+        out["rownorm_" + k] = np.linalg.norm(r[k], axis=0)
+        out["cols_" + k] = r[k][cols]
+    np.savez_compressed(os.path.join(HERE, "golden_fixed_dt_32x32x153.npz"), **out)
+
+
+def dg_golden():
+    """golden_dg.npz: the pressure matrices DG(3, 1:nyu, column) that the reference's LU_buildP + LU_decP
+    build (tridLU_3D.f90:224-297, 366-405; harness setup output), at 12x12x13 and 24x24x17."""
+    out = {}
+    for Mx, ny in ((12, 13), (24, 17)):
+        st = dns_ic.seeded_channel(Mx, Mx, ny, seed=1)
+        r = pyoracle.run_reference(Mx, Mx, ny, *st, mode=1, nsteps=0)
+        out["DG_%dx%d" % (Mx, ny)] = r["setup"]["DG"]
+    np.savez_compressed(os.path.join(HERE, "golden_dg.npz"), **out)
+
+
+def fixture32_state():
+    """fixture_32x32x153.npz: the reference's own 32x32x153 initial fields (input/*_0032x0032x0153_t00000.dat)
+    as column fields, with the header's t, so that the GPU tests start from them."""
+    u1, u2, u3, p, head = dns_io.load_state(REF_INPUT, "_0032x0032x0153_t00000.dat", 153)
+    np.savez_compressed(os.path.join(HERE, "fixture_32x32x153.npz"), u1=u1, u2=u2, u3=u3, p=p, t=head.t,
+                        iter=head.iter, Re=head.Re, mpgx=head.mpgx)
+
+
+ONLY = {"stats": lambda: stats_golden(), "ftp": lambda: ftp_golden(), "traj128": lambda: traj128_golden(),
+        "fixed_dt": lambda: fixed_dt_golden(), "dg": lambda: dg_golden(), "fixture32": lambda: fixture32_state()}
+
+
 def main():
     if not pyoracle.have_reference():
         raise SystemExit("oracle/_ref/ref_harness is not built")
-    if sys.argv[1:] == ["stats"]:
-        stats_golden()
-        return
-    if sys.argv[1:] == ["ftp"]:
-        ftp_golden()
+    if sys.argv[1:] and sys.argv[1] in ONLY:
+        for name in sys.argv[1:]:
+            ONLY[name]()
         return
     stats_golden()
     ftp_golden()
+    fixed_dt_golden()
+    dg_golden()
     Mx = Mz = 24
     ny = 17
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=7, amp=0.3, nmodes=4)
@@ -209,6 +299,8 @@ def main():
             out["norm_" + k] = np.linalg.norm(r[k])
             out["cols_" + k] = r[k][cols]
         np.savez_compressed(os.path.join(HERE, "golden_fixture32_steps.npz"), **out)
+        fixture32_state()
+    traj128_golden()
     print("golden fixtures written to", HERE)
 
 

@@ -97,10 +97,11 @@ def test_driver_snapshot_to_physical(tmp_path):
             ftp_compare(f.read(), want[n], 1e-13)
 
 
-def test_n_variant_on_driver_snapshot_stops_at_the_p_file(tmp_path):
-    """four_to_phys_N's p ranges read nyu + 1 planes from a file with record_out's current header (a
-    one-rank record_out writes that many); this build's snapshots hold nyu, so the N variant converts u1,
-    u2 and u3 and then stops with an end-of-file error on p, where the reference stops with a read error."""
+def test_n_variant_on_driver_snapshot(tmp_path):
+    """four_to_phys_N's p ranges read nyu + 1 planes from a file with record_out's current header.  A
+    one-rank record_out writes that many (limPL_incw(pgrid) = 1..nyu+1, start.f90:1187-1194; plane nyu+1
+    of ppPL is zero), and so does the drop-in driver, so the N variant converts all four fields; the
+    output files equal the oracle's restatement of four_to_phys_N on the same files."""
     from test_gpu_fortran import DRIVER, write_case
     Mx, Mz, ny = 48, 48, 33
     d = str(tmp_path)
@@ -108,12 +109,19 @@ def test_n_variant_on_driver_snapshot_stops_at_the_p_file(tmp_path):
     r = subprocess.run([DRIVER, "1", "fused"], cwd=d, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     Nx, Nz = 2 * Mx // 3, 2 * Mz // 3
+    outdir = os.path.join(d, "output")
+    files = {}
+    for n in FIELDS:
+        with open(os.path.join(outdir, "%s_%04dx%04dx%04d_t00000.dat" % (n, Nx, Nz, ny)), "rb") as f:
+            files[n] = f.read()
+    ff = dns_io.read_field(os.path.join(outdir, "p_%04dx%04dx%04d_t00000.dat" % (Nx, Nz, ny)))
+    assert list(ff.j) == list(range(1, ny + 1)) and not np.any(ff.planes[-1])   # p planes 1..nyu+1
     with open(os.path.join(d, "input_FtP.txt"), "w") as f:
         f.write("mgalx1=%10d\nmgalz1=%10d\nny=    %10d\nt=     %10s\n" % (Nx, Nz, ny, "0.0d0"))
     print("case: four_to_phys_hip N on a driver snapshot", flush=True)
     r = subprocess.run([PROG, "N"], cwd=d, capture_output=True, text=True, timeout=120)
-    assert r.returncode != 0 and "end of file" in r.stderr, r.stderr + r.stdout
-    outdir = os.path.join(d, "output")
-    for n in ("u1", "u2", "u3"):
-        path = os.path.join(outdir, "%s_phys_%04dx%04dx%04d_t00000.dat" % (n, Nx, Nz, ny))
-        assert os.path.getsize(path) > 0, n
+    assert r.returncode == 0, r.stderr + r.stdout
+    want = pyoracle.four_to_phys_files(files, True)
+    for n in FIELDS:
+        with open(os.path.join(outdir, "%s_phys_%04dx%04dx%04d_t00000.dat" % (n, Nx, Nz, ny)), "rb") as f:
+            ftp_compare(f.read(), want[n], 1e-13)

@@ -283,15 +283,65 @@ def test_oracle_vs_reference_live_seeded_32():
         assert rel_l2(a, r[k]) <= TOL, k
 
 
-@pytest.mark.skipif(not os.path.isdir("/root/reference/input"), reason="reference fixtures absent")
 def test_oracle_vs_reference_fixture32_golden(golden):
-    import dns_io
+    """The reference's own 32x32x153 initial fields (input/*_0032x0032x0153_t00000.dat, committed as
+    fixture_32x32x153.npz): 4 oracle steps against the reference's run of the same 4 steps."""
     g = golden("golden_fixture32_steps.npz")
-    u1, u2, u3, p, head = dns_io.load_state("/root/reference/input", "_0032x0032x0153_t00000.dat", 153)
+    fx = golden("fixture_32x32x153.npz")
+    u1, u2, u3, p = (fx[k] for k in ("u1", "u2", "u3", "p"))
     o = pyoracle.Oracle(48, 48, 153)
-    o.set_state(u1, u2, u3, p, t=head.t)
+    o.set_state(u1, u2, u3, p, t=float(fx["t"]))
     o.step(4)
     cols = g["cols"]
     for k, a in zip(("u1", "u2", "u3", "p"), o.get_state()):
         assert abs(np.linalg.norm(a) - g["norm_" + k]) <= TOL * g["norm_" + k], k
         assert rel_l2(a[cols], g["cols_" + k]) <= 1e-11, k
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/input"), reason="reference fixtures absent")
+def test_fixture32_npz_equals_reference_files(golden):
+    """The committed fixture holds exactly the fields of the reference's input files."""
+    import dns_io
+    fx = golden("fixture_32x32x153.npz")
+    u1, u2, u3, p, head = dns_io.load_state("/root/reference/input", "_0032x0032x0153_t00000.dat", 153)
+    for k, a in zip(("u1", "u2", "u3", "p"), (u1, u2, u3, p)):
+        assert np.array_equal(fx[k], a), k
+    assert float(fx["t"]) == head.t
+
+
+def test_oracle_DG_bit_exact_vs_reference(golden):
+    """setup_DG (the pressure matrices of LU_buildP + LU_decP, tridLU_3D.f90:224-297, 366-405) equals the
+    reference's own matrices bit for bit (golden_dg.npz, from the harness's set-up output)."""
+    g = golden("golden_dg.npz")
+    for Mx, ny in ((12, 13), (24, 17)):
+        DG = pyoracle.Oracle(Mx, Mx, ny).setup()["DG"].ravel()
+        assert np.array_equal(DG, g["DG_%dx%d" % (Mx, ny)]), (Mx, ny)
+
+
+def fixed_dt_errors(state, g):
+    """Largest relative error of the column norms, row norms and whole columns against the golden."""
+    err = 0.0
+    for k, a in zip(("u1", "u2", "u3", "p"), state):
+        cn, rn = g["colnorm_" + k], g["rownorm_" + k]
+        err = max(err, np.abs(np.linalg.norm(a, axis=1) - cn).max() / cn.max(),
+                  np.abs(np.linalg.norm(a, axis=0) - rn).max() / rn.max(),
+                  rel_l2(a[g["cols"]], g["cols_" + k]))
+    return err
+
+
+def test_oracle_fixed_dt_vs_reference_golden(golden):
+    """A run at fixed dt = 1e-3 (32x32x153, 10 steps from the seeded IC) against the reference's own code run
+    with its dt fixed the same way; a dt off by a relative 1e-9 must fail."""
+    g = golden("golden_fixed_dt_32x32x153.npz")
+    st = seeded(48, 48, 153, 1)
+    for k, a in zip(("u1", "u2", "u3", "p"), st):
+        assert abs(np.linalg.norm(a) - g["ic_norm_" + k]) <= 1e-15 * g["ic_norm_" + k], k
+    o = pyoracle.Oracle(48, 48, 153, fixed_dt=1e-3)
+    o.set_state(*st)
+    hist = o.step(10)
+    assert fixed_dt_errors(o.get_state(), g) <= TOL
+    assert np.array_equal(hist[:, 1], g["hist"][:, 2]) and np.allclose(hist[:, 0], g["hist"][:, 1], rtol=1e-15)
+    o = pyoracle.Oracle(48, 48, 153, fixed_dt=1e-3 * (1 + 1e-9))
+    o.set_state(*st)
+    o.step(10)
+    assert fixed_dt_errors(o.get_state(), g) > TOL
