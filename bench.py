@@ -6,7 +6,8 @@ and the reference's own CPU code timed on the host as a baseline.
 
 Prints one JSON line.  value = FP64 steps/s (the fields, the RK3 accumulation and the solves all FP64);
 the mixed-precision run (FP32 plane stage) is reported under "mixed" with its relative L2 velocity
-error against the FP64 run after the same number of steps from the same initial state.
+error against the FP64 run after the same number of steps from the same initial state, and after
+--error-steps (default 1000) steps beside the drift of an FP64 run from a 1e-10-perturbed state.
 """
 from __future__ import annotations
 
@@ -131,6 +132,23 @@ def cpu_all_cores_leg(st):
     t = time.perf_counter() - t0
     return {"value": 1.0 / t, "unit": "steps/s", "cores": n, "kind": "port",
             "sample": "one RK3 step of the 384x384x257 workload, OpenMP over %d threads, %.1f s" % (n, t)}
+
+
+def class_fracs(kms, fp32):
+    """HBM fraction of the plane stage (z-inverse + x-pass + z-forward) and of the column stage, each
+    charged its kernels' own compulsory bytes (dns_roofline.kernel_bytes) over their event-timed ms per
+    substep."""
+    import dns_roofline
+    by = dns_roofline.kernel_bytes(MX, MZ, NY, fp32)
+    plane_ms = sum(kms[k] for k in PLANE_KERNELS)
+    col = [k for k in kms if k not in PLANE_KERNELS]
+    col_ms = sum(kms[k] for k in col)
+    plane_b = sum(by[k] for k in PLANE_KERNELS)
+    col_b = sum(by[k] for k in col)
+    f = lambda b, ms: b / (ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS
+    return {"plane_stage": f(plane_b, plane_ms), "column": f(col_b, col_ms),
+            "plane_stage_GB_per_substep": plane_b / 1e9, "column_GB_per_substep": col_b / 1e9,
+            "plane_stage_ms_per_substep": plane_ms, "column_stage_ms_per_substep": col_ms}
 
 
 def traffic(kernel, prec_tag):
@@ -299,8 +317,9 @@ def main():
     ap.add_argument("--no-post", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-small", action="store_true")
-    ap.add_argument("--error-steps", type=int, default=0,
-                    help="also report the mixed path's error against FP64 after this many steps")
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="the mixed path's error against FP64 after this many steps, with the drift of a "
+                         "1e-10-perturbed FP64 run beside it (0: leave out)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box here has one MI355X); "
@@ -338,13 +357,15 @@ def main():
     ref64 = dns_roofline.reference_design_step_bytes(MX, MZ, NY, False)
     out["reference_design_step_GB"] = ref64 / 1e9
     out["step_frac_hbm_reference_design"] = ref64 / sec64 / 1e9 / dns_roofline.HBM_PEAK_GBS
+    out["class_frac_hbm"] = class_fracs(kms, False)
     cls = dns_roofline.reference_design_substep_classes(MX, MZ, NY, False)
     plane_ms = sum(kms[k] for k in PLANE_KERNELS)
     col_ms = sum(v for k, v in kms.items() if k not in PLANE_KERNELS)
     out["class_frac_hbm_reference_design"] = {
+        "note": "comparison only: the bytes the reference's own decomposition would move (its 12 transposes "
+                "per substep included), not what these kernels must move",
         "fft_and_transposes": cls["fft_and_transposes"] / (plane_ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
-        "column": cls["column"] / (col_ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS,
-        "plane_stage_ms_per_substep": plane_ms, "column_stage_ms_per_substep": col_ms}
+        "column": cls["column"] / (col_ms * 1e-3) / 1e9 / dns_roofline.HBM_PEAK_GBS}
     out["max_div"] = sc64["divmax"]
     out["autotune"] = {"fp64": tune64}
     if not args.no_mixed:
@@ -358,6 +379,7 @@ def main():
                         "step_frac_hbm_reference_design":
                             dns_roofline.reference_design_step_bytes(MX, MZ, NY, True) / secmx / 1e9
                             / dns_roofline.HBM_PEAK_GBS}
+        out["mixed"]["class_frac_hbm"] = class_fracs(timmx, True)
         clm = dns_roofline.reference_design_substep_classes(MX, MZ, NY, True)
         pm = sum(timmx[k] for k in PLANE_KERNELS)
         cm = sum(v for k, v in timmx.items() if k not in PLANE_KERNELS)
