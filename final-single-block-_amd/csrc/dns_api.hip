@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1079,6 +1080,54 @@ extern "C" int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, con
   (void)hipFree(dmid);
   (void)hipFree(tw);
   return rc;
+}
+
+// The same per-plane work on device memory the caller owns, with no device allocation: the z-inverted
+// columns and the twiddle tables live in the library's static workspace (dnsk::FTP_SCRATCH_BYTES per GPU,
+// post_kernels.hip), so planes are converted in batches that fit it.  Same kernels and twiddles as
+// dns_four_to_phys, so the same bits; only the row pitches differ (the caller's dense rows).
+extern "C" int dns_four_to_phys_noalloc(int Nx, int Nz, int Mx, int Mz, int nplanes, const double *d_in,
+                                        double *d_out, void *stream) {
+  if (nplanes < 0 || (nplanes > 0 && (!d_in || !d_out)))
+    return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: bad argument");
+  if (Nx < 2 || Nz < 2 || Nx % 2 || Nz % 2 || Mx < Nx || Mz < Nz)
+    return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: sizes %d x %d -> %d x %d (need even N <= M)", Nx, Nz, Mx,
+                Mz);
+  if (!dnsk::four_to_phys_supported(Mx, Mz))
+    return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: %d x %d is not a supported transform size (Mz and Mx/2 "
+                "in 8..1152, 2^a 3^b)", Mx, Mz);
+  if (((uintptr_t)d_in | (uintptr_t)d_out) % 16 != 0)
+    return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: d_in and d_out must be 16-byte aligned");
+  if (nplanes == 0) return DNS_OK;
+  void *base = nullptr;
+  if (dnsk::four_to_phys_scratch(&base))
+    return fail(DNS_ERR_HIP, "dns_four_to_phys_noalloc: no address for the workspace on this device");
+  const int mp = (Nx / 2 + 1 + 7) / 8 * 8;                                  // complex per intermediate row
+  const size_t ntw = (size_t)Mz + Mx + Mx / 2;                              // twiddles, complex
+  const size_t midoff = (ntw * sizeof(double2) + 255) / 256 * 256;          // intermediate after them
+  const size_t per = (size_t)Mz * mp * sizeof(double2);                     // intermediate of one plane
+  if (midoff + per > dnsk::FTP_SCRATCH_BYTES)
+    return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: %d x %d -> %d x %d needs %zu bytes of workspace for one "
+                "plane; the workspace holds %zu", Nx, Nz, Mx, Mz, midoff + per, dnsk::FTP_SCRATCH_BYTES);
+  const int nb = (int)std::min<size_t>((size_t)nplanes, (dnsk::FTP_SCRATCH_BYTES - midoff) / per);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double2 *tw = static_cast<double2 *>(base);
+  double2 *mid = reinterpret_cast<double2 *>(static_cast<char *>(base) + midoff);
+  const auto a = twiddles<double>(Mz), b = twiddles<double>(Mx), c = twiddles<double>(Mx / 2);
+  HIPCHK(hipMemcpyAsync(tw, a.data(), Mz * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(tw + Mz, b.data(), Mx * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(tw + Mz + Mx, c.data(), (Mx / 2) * sizeof(double2), hipMemcpyHostToDevice, st));
+  const size_t hin = (size_t)(Nx + 2) * Nz, hout = (size_t)(Mx + 2) * Mz;   // dense planes
+  for (int p0 = 0; p0 < nplanes; p0 += nb) {
+    const int n = std::min(nb, nplanes - p0);
+    dnsk::FtpArgs fa{Nx, Nz, Mx, Mz, n, Nx + 2, mp, Mx + 2, d_in + (size_t)p0 * hin, mid,
+                     d_out + (size_t)p0 * hout, tw, tw + Mz, tw + Mz + Mx};
+    if (dnsk::launch_four_to_phys(fa, st))
+      return fail(DNS_ERR_ARG, "dns_four_to_phys_noalloc: %d x %d not instantiated", Mx, Mz);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));  // the host twiddle tables are read before they go out of scope
+  return DNS_OK;
 }
 
 extern "C" int dns_set_column_path(dns_handle *h, int path) {

@@ -201,6 +201,10 @@ struct FtpArgs {
 };
 bool four_to_phys_supported(int Mx, int Mz);
 int launch_four_to_phys(const FtpArgs &a, hipStream_t st);
+// the allocation-free entry point's workspace (post_kernels.hip): a static device array of this size;
+// *ptr = its address on the current device (0), or -1 when the address cannot be had
+constexpr size_t FTP_SCRATCH_BYTES = (size_t)16 << 20;
+int four_to_phys_scratch(void **ptr);
 
 // one-pass column stage (column_pass.hip): flag_ctpress = 1, 8 <= nyu <= 512
 bool column_pass_supported(int nyu);

@@ -12,9 +12,33 @@
 //   ftp_x_kernel: the unnormalised real inverse of length Mx of every row (rft, iopt = 1) as a complex
 //     transform of length Mx/2 after the usual even/odd split: z_n = (c_n + conj c_{H-n}) +
 //     i (c_n - conj c_{H-n}) e^{+2 pi i n/Mx}, c_0 and c_H taken real, u[2n] = Re z_n, u[2n+1] = Im z_n.
+#include <mutex>
+
 #include "dns_kernels.h"
 
 namespace dnsk {
+
+// Workspace of the allocation-free four_to_phys entry point: the z-inverted columns of a batch of planes
+// and the twiddle tables, in a file-scope device array (each GPU holds its own copy until the process
+// ends).  Host code never passes the array's name to a kernel or a copy: in host code it is a stand-in,
+// not a device address.  Each device's address comes from hipGetSymbolAddress, taken once per device
+// with that device current (four_to_phys_scratch).
+static __device__ __align__(256) double2 scratch[FTP_SCRATCH_BYTES / sizeof(double2)];
+
+int four_to_phys_scratch(void **ptr) {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  return 0;
+}
 
 // z-tile: T columns of one plane per workgroup (256-byte row segments where LDS allows), T * MZ complex
 // in LDS; the 12.12.x plans of 576 and 1152 take 48 threads per column (one butterfly per thread and pass)

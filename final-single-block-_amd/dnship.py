@@ -46,7 +46,7 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
            "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
            "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats",
-           "dns_four_to_phys"]
+           "dns_four_to_phys", "dns_four_to_phys_noalloc"]
 
 
 def lib():
@@ -95,6 +95,7 @@ def lib():
     L.dns_reset_stats.argtypes = [vp]
     L.dns_get_stats.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.dns_four_to_phys.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp]
+    L.dns_four_to_phys_noalloc.argtypes = [ctypes.c_int] * 5 + [vp, vp, vp]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -127,6 +128,12 @@ def four_to_phys(planes, Mx, Mz):
     ms = ctypes.c_double()
     _chk(lib().dns_four_to_phys(nx - 2, Nz, Mx, Mz, n, _p(planes), _p(out), ctypes.byref(ms)))
     return out, ms.value
+
+def four_to_phys_noalloc(d_in, d_out, n, Nx, Nz, Mx, Mz, stream=None):
+    """dns_four_to_phys_noalloc on device pointers (ints): n dense planes (Nx+2) x Nz at d_in to n dense
+    physical planes (Mx+2) x Mz at d_out, with no device allocation (the library's static workspace)."""
+    _chk(lib().dns_four_to_phys_noalloc(int(Nx), int(Nz), int(Mx), int(Mz), int(n), d_in, d_out, stream))
+
 
 class DNS:
     """One channel on the GPU (one dns_handle)."""

@@ -195,6 +195,13 @@ module dns_hip
       real(c_double), intent(out) :: fout(*)
       real(c_double), intent(out) :: ms
     end function
+    ! device pointers (type(c_ptr)) and a stream (c_null_ptr: the null stream); no device allocation
+    integer(c_int) function dns_four_to_phys_noalloc(Nx, Nz, Mx, Mz, nplanes, d_in, d_out, stream) &
+        bind(C, name='dns_four_to_phys_noalloc')
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nx, Nz, Mx, Mz, nplanes
+      type(c_ptr), value :: d_in, d_out, stream
+    end function
     integer(c_int) function dns_set_column_path(h, path) bind(C, name='dns_set_column_path')
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -168,6 +168,17 @@ int dns_reset_stats(dns_handle *h);
  * and kept when Mz == Nz (four_to_phys_N, Ngal = N).  ms (may be NULL): device time of the transform
  * kernels in ms. */
 int dns_four_to_phys(int Nx, int Nz, int Mx, int Mz, int nplanes, const double *in, double *out, double *ms);
+/* dns_four_to_phys on device memory, with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  d_in: nplanes dense spectral planes (Nx+2) x Nz; d_out: nplanes dense physical
+ * planes (Mx+2) x Mz; both device memory, 16-byte aligned; stream: a hipStream_t (NULL: the null stream);
+ * returns after the work on that stream has finished.  The intermediate and the twiddle tables live in the
+ * library's static workspace of 16 MiB per GPU (held until the process ends); planes go through it in
+ * batches, and sizes whose single plane does not fit are rejected with DNS_ERR_ARG.  Every call on one GPU
+ * shares that GPU's workspace: calls that overlap on one GPU (other threads, other streams) race.  Results
+ * equal dns_four_to_phys bit for bit. */
+int dns_four_to_phys_noalloc(int Nx, int Nz, int Mx, int Mz, int nplanes, const double *d_in, double *d_out,
+                             void *stream);
 
 /* Column stage of the fused path: path 0 = one kernel per substep (rows split into segments, tridiagonal
  * systems by the partition method, everything read and written once), 1 = three sweeps (thread per
