@@ -9,8 +9,9 @@
 //   solveP                    littleharsh.f90:626-680, tridLU_3D.f90:98-132, 224-297
 //   v_corr (+ max|div|)       littleharsh.f90:896-993
 //   meanflow_ctP (kRK = 3)    littleharsh.f90:572-589
-// for the constant pressure gradient case (flag_ctpress = 1); the constant flow rate correction
-// (meanflow_ctU) needs the solved mean mode between the solves and runs on the three-sweep path.
+// With a constant flow rate (flag_ctpress = 0) the correction of meanflow_ctU follows this kernel in
+// meanflow_post_kernel (column_kernels.hip): it changes only mode (0,0) of u1, which the pressure solve
+// and v_corr do not read, so applying it after them gives the reference's result.
 //
 // Layout of the work.  Rows 1..P*M of every column are split into P segments of M rows; thread
 // (c, s) = (tid % NC, tid / NC) owns rows 1 + s*M .. (s+1)*M of column c, so NC consecutive threads

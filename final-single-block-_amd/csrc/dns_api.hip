@@ -238,7 +238,7 @@ extern "C" int dns_create(const dns_params *prm, dns_handle **out) {
   h->nyv = ny - 2; h->nyu = h->nyv + 1;
   h->fp32 = prm->precision == DNS_PREC_MIXED;
   h->nm = prm->members > 1 ? prm->members : 1;
-  h->colpass = prm->column_path == 0 && prm->flag_ctpress == 1 && dnsk::column_pass_supported(h->nyu);
+  h->colpass = prm->column_path == 0 && dnsk::column_pass_supported(h->nyu);
   h->uploaded.assign(h->nm, 0);
   h->Re_m.assign(h->nm, prm->Re);
   h->mpgx_m.assign(h->nm, prm->mpgx);
@@ -745,6 +745,9 @@ static int fused_substep(dns_handle *h, int kRK) {
   if (kRK != 1) dnsk::launch_divmax_reset(h->st, h->nm, h->stream);
   if (h->colpass) {
     if (dnsk::launch_column_pass(a, h->stream)) return fail(DNS_ERR_ARG, "column pass: %d rows not instantiated", h->nyu);
+    // constant flow rate: meanflow_ctU on mode (0,0), which commutes with solveP and v_corr there
+    if (h->prm.flag_ctpress == 0 && dnsk::launch_meanflow_post(a, h->stream))
+      return fail(DNS_ERR_ARG, "column pass: %d rows exceed the mean-flow kernel", h->nyu);
     tmark(h, 4);
     tmark(h, 5);
     tmark(h, 6);
@@ -1134,7 +1137,7 @@ extern "C" int dns_set_column_path(dns_handle *h, int path) {
   if (!h) return fail(DNS_ERR_ARG, "dns_set_column_path: null handle");
   if (path != 0 && path != 1) return fail(DNS_ERR_ARG, "dns_set_column_path: path=%d must be 0 or 1", path);
   h->prm.column_path = path;
-  h->colpass = path == 0 && h->prm.flag_ctpress == 1 && dnsk::column_pass_supported(h->nyu);
+  h->colpass = path == 0 && dnsk::column_pass_supported(h->nyu);
   drop_graph(h);
   return DNS_OK;
 }

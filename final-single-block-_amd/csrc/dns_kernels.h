@@ -167,6 +167,8 @@ void launch_rhs0(const ColArgs &a, hipStream_t st);
 void launch_nl_col(const ColArgs &a, hipStream_t st);
 void launch_solveU(const ColArgs &a, hipStream_t st);
 void launch_meanflow(const ColArgs &a, hipStream_t st);
+// meanflow_ctU after the one-pass column stage (solveP and v_corr already done); -1 when nyu > 512
+int launch_meanflow_post(const ColArgs &a, hipStream_t st);
 void launch_solveP(const ColArgs &a, hipStream_t st);
 void launch_vcorr(const ColArgs &a, hipStream_t st);
 void launch_qx(const ColArgs &a, hipStream_t st);
@@ -206,7 +208,8 @@ int launch_four_to_phys(const FtpArgs &a, hipStream_t st);
 constexpr size_t FTP_SCRATCH_BYTES = (size_t)16 << 20;
 int four_to_phys_scratch(void **ptr);
 
-// one-pass column stage (column_pass.hip): flag_ctpress = 1, 8 <= nyu <= 512
+// one-pass column stage (column_pass.hip): 8 <= nyu <= 512 (with flag_ctpress = 0 followed by
+// launch_meanflow_post)
 bool column_pass_supported(int nyu);
 int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered
 // layout

@@ -50,8 +50,9 @@ typedef struct dns_params {
                              an even number); 0 = all planes */
   int members;            /* ensemble members of these sizes (0 or 1: a single channel); each member has
                              its own fields, Re, mpgx, dt and state, and every launch covers all members */
-  int column_path;        /* column stage: 0 one kernel per substep (held on chip; used when flag_ctpress = 1
-                             and 8 <= ny-1 <= 512, else the sweeps), 1 three sweeps */
+  int column_path;        /* column stage: 0 one kernel per substep (held on chip; used when
+                             8 <= ny-1 <= 512, else the sweeps; with flag_ctpress = 0 a one-workgroup
+                             kernel applies meanflow_ctU after it), 1 three sweeps */
   int reserved[5];
 } dns_params;
 

@@ -320,18 +320,19 @@ def test_autotune_choice_valid_and_results_unchanged():
 
 
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
-@pytest.mark.parametrize("Mx,Mz,ny,ctp", [(48, 48, 257, 1), (96, 96, 129, 1), (24, 24, 33, 0)])
+@pytest.mark.parametrize("Mx,Mz,ny,ctp", [(48, 48, 257, 1), (96, 96, 129, 1), (24, 24, 33, 0), (48, 48, 257, 0),
+                                          (24, 24, 7, 0)])
 def test_column_paths_agree(prec, Mx, Mz, ny, ctp):
     """One-pass column stage against the three sweeps on one handle (same plane stage); the one-pass
-    kernel is used where it applies (constant pressure gradient, 8 <= ny-1 <= 512), else both are the
-    sweeps and agree bit for bit."""
+    kernel is used where it applies (8 <= ny-1 <= 512; with a constant flow rate followed by the mean-flow
+    correction), else both are the sweeps and agree bit for bit."""
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=11, broadband=True)
     print("case: column paths %dx%dx%d ctp=%d prec %d" % (Mx, Mz, ny, ctp, prec), flush=True)
     outs = []
     for path in (0, 1):
         g = dnship.DNS(Mx, Mz, ny, precision=prec, flag_ctpress=ctp, column_path=path)
         used = g.column_path()
-        assert used == (path if (ctp == 1 and 8 <= ny - 1 <= 512) else 1)
+        assert used == (path if 8 <= ny - 1 <= 512 else 1)
         g.upload(*st)
         g.step(3)
         g.sync()
@@ -345,6 +346,12 @@ def test_column_paths_agree(prec, Mx, Mz, ny, ctp):
         assert rel_l2(x, y) <= tol
     assert abs(sa["t"] - sb["t"]) <= 1e-14 * sb["t"] and sa["iter"] == sb["iter"] == 3
     assert abs(sa["Qx"] - sb["Qx"]) <= 1e-12 * abs(sb["Qx"])
+    # g = dgx = (QxT - Qx0)/Qx1 divides the difference of two nearly equal flow rates by the small flow
+    # rate Qx1 of the unit response (O(dt)), so the paths' rounding differences in u1(0,0) (1e-13 of Qx0)
+    # reach g and mpgx amplified: both are compared on the scale of mpgx, not of the tiny correction g
+    tg = 1e-9 if prec == dnship.PREC_FP64 else 1e-6
+    assert abs(sa["mpgx"] - sb["mpgx"]) <= tg * abs(sb["mpgx"])
+    assert abs(sa["dgx"] - sb["dgx"]) <= tg * abs(sb["mpgx"])
     assert sa["divmax"] < 1e-9 and sb["divmax"] < 1e-9
 
 
@@ -368,8 +375,11 @@ def test_history_record_vs_oracle(ctp):
     u1 = o.get_state()[0]
     _, div = o.get_psi()
     assert h["flag_ctpress"] == ctp and h["iter"] == n
-    for k in ("t", "dt", "dtc", "mpgx"):
+    for k in ("t", "dt", "dtc"):
         assert abs(h[k] - sc[k]) <= 1e-13 * abs(sc[k]), k
+    # constant flow rate: mpgx accumulates g = (QxT - Qx0)/Qx1, a difference quotient that amplifies the
+    # rounding of the flow rate (see test_column_paths_agree); constant pressure gradient: mpgx is input
+    assert abs(h["mpgx"] - sc["mpgx"]) <= (1e-9 if ctp == 0 else 1e-13) * abs(sc["mpgx"])
     # record_out recomputes flowrateIm from the current u1 (littleharsh.f90:473-492); with a constant
     # pressure gradient that is the oracle's own Qx of meanflow_ctP
     yv = su["yv"]
@@ -379,7 +389,8 @@ def test_history_record_vs_oracle(ctp):
     assert abs(h["Qx"] - qx) <= 1e-13 * abs(qx)
     assert abs(h["dtv"] - su["dtv"]) <= 1e-15 * su["dtv"]
     if ctp == 0:
-        assert abs(h["QxT"] - sc["QxT"]) <= 1e-13 * abs(sc["QxT"]) and abs(h["dgx"] - sc["dgx"]) <= 1e-12
+        assert abs(h["QxT"] - sc["QxT"]) <= 1e-13 * abs(sc["QxT"])
+        assert abs(h["dgx"] - sc["dgx"]) <= 1e-9 * abs(sc["mpgx"])
     assert h["err"] < 1e-9 and abs(h["err"] - np.abs(div).max()) <= 1e-12
     c0 = u1[0].real                       # mode (0,0), rows 0..nyu+1
     yu = su["yu"]
