@@ -71,38 +71,72 @@ ENSEMBLE = [(24, 48, 48, 153), (16, 96, 96, 97), (8, 192, 192, 153)]
 
 
 def ensemble_leg(steps, warmup, prec):
+    """The 48-member ensemble (three sizes) with one grouped launch per kernel class per RK3 substep for
+    all members, the step replayed from a captured hipGraph (value); beside it the same members as three
+    handles on three streams, each handle's step autotuned and replayed from its own graph."""
     import numpy as np
     import dnship
     import dns_ic
     import dns_roofline
     rng = np.random.default_rng(7)
-    handles, by, idx = [], 0.0, 0
+    handles, by, idx, ics = [], 0.0, 0, []
     for nm, Mx, Mz, ny in ENSEMBLE:
         g = dnship.DNS(Mx, Mz, ny, precision=prec, members=nm)
+        sts = []
         for m in range(nm):
             ret = rng.uniform(150.0, 400.0)
             Re = 0.5 * (ret / 0.09) ** (1.0 / 0.88)
             g.set_member_params(m, Re, -(ret / Re) ** 2)
-            g.upload(*dns_ic.seeded_channel(Mx, Mz, ny, seed=100 + idx, amp=0.1), member=m)
+            st = dns_ic.seeded_channel(Mx, Mz, ny, seed=100 + idx, amp=0.1)
+            g.upload(*st, member=m)
+            sts.append(st)
             idx += 1
-        g.autotune(iters=3)
         handles.append(g)
+        ics.append(sts)
         by += nm * dns_roofline.kernel_bytes(Mx, Mz, ny, prec == dnship.PREC_MIXED)["step"]
-    for g in handles:          # each handle's RK3 step is captured once in a hipGraph and replayed
+
+    def reupload():
+        for g, sts in zip(handles, ics):
+            for m, st in enumerate(sts):
+                g.upload(*st, member=m)
+
+    ens = dnship.Ensemble(handles)
+    ens.step_graph(max(warmup, 1))
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step_graph(steps)
+    ens.sync()
+    sec = (time.perf_counter() - t0) / steps
+    reupload()
+    ens.set_timing(1)
+    ens.step(2)
+    ens.sync()
+    tim = ens.timing()
+    ens.close()
+    # the same members as three handles on three streams, each autotuned, each step from its own graph
+    for g in handles:
+        g.autotune(iters=3)
+    reupload()
+    for g in handles:
         g.step_graph(max(warmup, 1))
     for g in handles:
         g.sync()
     t0 = time.perf_counter()
-    for g in handles:          # one stream per handle: the three groups run concurrently
+    for g in handles:
         g.step_graph(steps)
     for g in handles:
         g.sync()
-    sec = (time.perf_counter() - t0) / steps
+    sec_streams = (time.perf_counter() - t0) / steps
     for g in handles:
         g.close()
     return {"value": 1.0 / sec, "unit": "ensemble steps/s (all 48 members advance one RK3 step)",
             "ms_per_step": sec * 1e3, "members": idx, "groups": [list(x) for x in ENSEMBLE],
-            "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS}
+            "launch": "one grouped launch per kernel class per substep for all members, step replayed from a "
+                      "hipGraph",
+            "kernels_ms_per_substep": {k: v for k, v in tim.items() if k != "n"},
+            "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
+            "three_streams": {"value": 1.0 / sec_streams, "ms_per_step": sec_streams * 1e3,
+                              "launch": "one handle per size on its own stream, autotuned, own hipGraph"}}
 
 
 def cpu_leg(st):

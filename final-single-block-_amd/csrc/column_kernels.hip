@@ -187,8 +187,8 @@ __global__ void meanflow_kernel(ColArgs a_) {
 // matrix rows, yv and u1's mode (0,0) in LDS; one thread runs LUsol0's sweeps in the reference's order
 // (tridLU_3D.f90:134-160, 299-325) and the two flow-rate sums.
 constexpr int MEANFLOW_MAXN = 516;  // nyu + 2 rows, nyu <= 512 (the one-pass column stage's range)
-__global__ void __launch_bounds__(256) meanflow_post_kernel(ColArgs a_) {
-  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
+__device__ __forceinline__ void meanflow_post_body(const ColArgs &a_, int member) {
+  const ColArgs a = col_view(a_, member);
   __shared__ double c1[MEANFLOW_MAXN], c2[MEANFLOW_MAXN], c3[MEANFLOW_MAXN], yv[MEANFLOW_MAXN],
       ur[MEANFLOW_MAXN], u11[MEANFLOW_MAXN], w3[MEANFLOW_MAXN];
   __shared__ double gsh;
@@ -252,6 +252,27 @@ __global__ void __launch_bounds__(256) meanflow_post_kernel(ColArgs a_) {
     a.u1[0] = rmul(a.gbc, d2(ur[1], a.u1[C].y));
     a.u1[(size_t)(a.nyu + 1) * C] = rmul(a.gbc, d2(ur[a.nyu], a.u1[(size_t)a.nyu * C].y));
   }
+}
+__global__ void __launch_bounds__(256) meanflow_post_kernel(ColArgs a) { meanflow_post_body(a, blockIdx.y); }
+
+// one block per member of the groups with flags[g] = 1 (constant flow rate)
+__global__ void __launch_bounds__(256) ens_meanflow_post_kernel(EnsColLaunch E, int4 flags) {
+  const int f[4] = {flags.x, flags.y, flags.z, flags.w};
+  int m = blockIdx.x, g = 0;
+  for (; g < E.ng; g++) {
+    const int n = f[g] ? E.a[g].nm : 0;
+    if (m < n) break;
+    m -= n;
+  }
+  if (g < E.ng) meanflow_post_body(E.a[g], m);
+}
+void launch_ens_meanflow_post(const EnsColLaunch &E, const int *flags, hipStream_t st) {
+  static_assert(ENS_MAXG == 4, "flags travel as an int4");
+  int total = 0;
+  for (int g = 0; g < E.ng; g++) total += flags[g] ? E.a[g].nm : 0;
+  if (total == 0) return;
+  const int4 f = make_int4(flags[0], E.ng > 1 ? flags[1] : 0, E.ng > 2 ? flags[2] : 0, E.ng > 3 ? flags[3] : 0);
+  ens_meanflow_post_kernel<<<total, 256, 0, st>>>(E, f);
 }
 
 // flowrateIm of mode (0,0) : littleharsh.f90:473-492, meanflow_ctP :572-589

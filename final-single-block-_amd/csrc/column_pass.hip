@@ -227,29 +227,49 @@ struct NA {
   __device__ __forceinline__ void set(size_t i, double2 v) const { static_cast<cx<T> *>(p)[i] = from_d2<T>(v); }
 };
 
-template <int NC, int P, int M, typename NT>
-__global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(ColArgs a_) {
-  constexpr int NTH = NC * P;
-  __shared__ PassShared<NC, P> sh;
-  __shared__ double u1q[P * M];  // column 0's u1 for the flow rate
+// The kernel is a body (LDS layout + run(), block given by the caller) and a thin __global__ wrapper; the
+// ensemble's grouped launch (ensemble.hip) runs bodies of several row counts in one launch.
+template <int NC, int P, int M>
+struct PassLds {
   // The segment's p rows from the first phase, kept in LDS for p += psi at the end instead of being read
   // from HBM again: with 64 segments (one workgroup per CU) where LDS has room; with fewer segments the
   // stash made the compiler spill.
-  constexpr bool PST = P >= 64 && sizeof(PassShared<NC, P>) + sizeof(double) * P * M +
-                                       sizeof(double2) * M * NC * P <= 160 * 1024;
-  __shared__ double2 pst[PST ? M * NC * P : 1];
-  const ColArgs a = col_view(a_, blockIdx.y);  // ensemble member
+  static constexpr bool PST = P >= 64 && sizeof(PassShared<NC, P>) + sizeof(double) * P * M +
+                                             sizeof(double2) * M * NC * P <= 160 * 1024;
+  PassShared<NC, P> sh;
+  double u1q[P * M];  // column 0's u1 for the flow rate
+  double2 pst[PST ? M * NC * P : 1];
+};
+
+template <int NC, int P, int M, typename NT>
+struct ColumnPassBody {
+  static constexpr int NTH = NC * P;
+  static constexpr size_t LDS = sizeof(PassLds<NC, P, M>);
+  // b: column block of member my (nblk column blocks per member); tid: thread index in the body's NTH
+  // threads (a grouped launch runs several bodies side by side in one workgroup, all of them through the
+  // same barriers); active = false: a padding body that computes and stores nothing
+  __device__ __forceinline__ static void run(const ColArgs &a_, int b, int nblk, int my, int tid,
+                                             unsigned char *lds, bool active = true);
+};
+
+template <int NC, int P, int M, typename NT>
+__device__ __forceinline__ void ColumnPassBody<NC, P, M, NT>::run(const ColArgs &a_, int b, int nblk, int my,
+                                                                  int tid, unsigned char *lds, bool active) {
+  PassLds<NC, P, M> &L = *reinterpret_cast<PassLds<NC, P, M> *>(lds);
+  PassShared<NC, P> &sh = L.sh;
+  double *u1q = L.u1q;
+  double2 *pst = L.pst;
+  constexpr bool PST = PassLds<NC, P, M>::PST;
+  const ColArgs a = col_view(a_, my);  // ensemble member
   const NA<NT> Nu1{a.Nu1.p}, Nu2{a.Nu2.p}, Nu3{a.Nu3.p}, Nxz1{a.Nxz1.p}, Nxz2{a.Nxz2.p}, Nxz3{a.Nxz3.p};
   const NA<NT> uvf{a.uvf.p}, vvc{a.vvc.p}, wvf{a.wvf.p};
-  const int tid = threadIdx.x;
   const int c = tid % NC, s = tid / NC;
   const int C = a.C, nyu = a.nyu, nyv = a.nyv;
   // Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous run of column blocks, so
   // that blocks sharing a 128-byte line of a row (NC = 4) are read through the same L2.
-  const int nblk = gridDim.x, b = blockIdx.x;
   const int cb = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
   const int col0 = cb * NC + c;
-  const bool live = col0 < C;
+  const bool live = active && col0 < C;
   const int col = live ? col0 : C - 1;  // threads past the last column redo its work and store nothing
   const Mode m = col_mode(a, col);
   const double dt = substep_dt(a, (live && s == 0) ? col : -1);
@@ -423,7 +443,7 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
   // ---- p += psi, v_corr, wall rows; corrected values stay in su / sv for max|div| --------------
   const double dtrk = dt * (a.aRK + a.bRK);
   const double kxd = dtrk * m.kx, kzd = dtrk * m.kz;
-  const bool qx = cb == 0 && a.kRK == 3 && a.flag_ctpress == 1;
+  const bool qx = active && cb == 0 && a.kRK == 3 && a.flag_ctpress == 1;
   {
     const double2 psn_top = s == P - 1 ? z2 : sh.x1[(s + 1) * NC + c];
     const int jb = opaque(j0), cl = opaque(col);
@@ -483,7 +503,7 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
     }
   }
   if (!live) dmax = 0.0;
-  dmax = block_max<NTH>(dmax, sh.red);
+  dmax = block_max_sub<NTH>(dmax, sh.red, tid);
   if (tid == 0) {
     atomic_max_nonneg(&a.st->divmax, dmax);
     if (qx) {  // flowrateIm of mode (0,0) in the reference's summation order (littleharsh.f90:473-492)
@@ -491,15 +511,22 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
       for (int j = 1; j <= nyv + 1; j++) Q = Q + (a.yv[j] - a.yv[j - 1]) * u1q[j - 1];
       a.st->Qx = Q;
     }
-    if (cb == 0 && a.kRK == 3) a.st->iter = a.st->iter + 1.0;
+    if (active && cb == 0 && a.kRK == 3) a.st->iter = a.st->iter + 1.0;
   }
+}
+
+template <int NC, int P, int M, typename NT>
+__global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(ColArgs a) {
+  using B = ColumnPassBody<NC, P, M, NT>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  B::run(a, blockIdx.x, gridDim.x, blockIdx.y, threadIdx.x, lds);
 }
 
 // (NC, P, M) by the rows to cover (nyu <= P*M, M >= 3: a segment needs a first, an interior and a last
 // row).  M = 4 keeps a thread within 256 registers (two wavefronts per SIMD); longer segments spill and
 // run one wavefront per SIMD.  Each bucket's shape was timed against alternatives (3-row segments at 96
 // rows, 16 columns x 32 x 5 at the 153-point grids, 8 columns x 64 x 6 at 385 points).
-int column_pass_rows(int nyu) {
+__host__ __device__ int column_pass_rows(int nyu) {
   return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 96 ? 96 : nyu <= 128 ? 128 : nyu <= 160 ? 160
          : nyu <= 192 ? 192 : nyu <= 256 ? 256 : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
 }
@@ -526,6 +553,77 @@ int launch_column_pass(const ColArgs &a, hipStream_t st) {
     case 512: launch_pass<4, 64, 8>(a, st); break;
     default: return 1;
   }
+  return 0;
+}
+
+// ---- grouped launch of the ensemble: every member of every size group in one launch -----------------
+// A 512-thread workgroup runs 512/(NC*P) bodies of the group's single-channel shape side by side (each with
+// its own slice of LDS and its own column block), so every member gets exactly the arithmetic of the
+// single-channel launch; the shapes of rows <= 160 take 64 to 512 threads and about 116 KB of LDS per
+// 512 threads.  Row buckets as column_pass_rows: 32 (8 x 8 x 4), 64 (8 x 16 x 4), 96 (8 x 32 x 3),
+// 128 (8 x 32 x 4), 160 (16 x 32 x 5).
+constexpr int ENS_COL_NTH = 512;
+template <int NC, int P, int M, typename NT>
+struct EnsColShape {
+  using B = ColumnPassBody<NC, P, M, NT>;
+  static constexpr int SUB = ENS_COL_NTH / B::NTH;   // bodies per workgroup
+  static constexpr size_t LDS = SUB * ((B::LDS + 15) / 16 * 16);
+};
+template <typename NT> constexpr size_t ens_col_lds() {
+  constexpr size_t l[5] = {EnsColShape<8, 8, 4, NT>::LDS, EnsColShape<8, 16, 4, NT>::LDS,
+                           EnsColShape<8, 32, 3, NT>::LDS, EnsColShape<8, 32, 4, NT>::LDS,
+                           EnsColShape<16, 32, 5, NT>::LDS};
+  size_t m = 0;
+  for (size_t v : l) m = v > m ? v : m;
+  return m;
+}
+
+bool ens_column_rows_supported(int nyu) { return nyu >= 8 && nyu <= 160; }
+
+template <int NC, int P, int M, typename NT>
+__device__ __forceinline__ void ens_col_run(const ColArgs &a, int lb, int nb, unsigned char *lds) {
+  using S = EnsColShape<NC, P, M, NT>;
+  const int sub = threadIdx.x / S::B::NTH, tid = threadIdx.x % S::B::NTH;
+  // workgroup lb holds column blocks lb*SUB .. lb*SUB + SUB-1 of the group (member-major); a body past the
+  // group's last block computes the last block again (every barrier is reached) and stores nothing
+  const int gb = lb * S::SUB + sub;
+  const int total = nb * a.nm;
+  const bool active = gb < total;
+  const int b = active ? gb : total - 1;
+  S::B::run(a, b % nb, nb, b / nb, tid, lds + sub * ((S::B::LDS + 15) / 16 * 16), active);
+}
+
+template <typename NT>
+__global__ void __launch_bounds__(ENS_COL_NTH, 1) ens_column_pass_kernel(EnsColLaunch E) {
+  __shared__ __align__(16) unsigned char lds[ens_col_lds<NT>()];
+  int g = 0;
+  const int b = blockIdx.x;
+  while (g + 1 < E.ng && b >= E.off[g + 1]) g++;
+  const int lb = b - E.off[g], nb = E.nblk[g];
+  switch (column_pass_rows(E.a[g].nyu)) {
+    case 32: ens_col_run<8, 8, 4, NT>(E.a[g], lb, nb, lds); break;
+    case 64: ens_col_run<8, 16, 4, NT>(E.a[g], lb, nb, lds); break;
+    case 96: ens_col_run<8, 32, 3, NT>(E.a[g], lb, nb, lds); break;
+    case 128: ens_col_run<8, 32, 4, NT>(E.a[g], lb, nb, lds); break;
+    default: ens_col_run<16, 32, 5, NT>(E.a[g], lb, nb, lds); break;
+  }
+}
+
+int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st) {
+  int total = 0;
+  for (int g = 0; g < E.ng; g++) {
+    const int nyu = E.a[g].nyu;
+    if (!ens_column_rows_supported(nyu)) return -1;
+    const int rows = column_pass_rows(nyu);
+    const int nc = rows == 160 ? 16 : 8;
+    const int sub = rows == 32 ? 8 : rows == 64 ? 4 : rows == 160 ? 1 : 2;   // bodies per workgroup
+    E.nblk[g] = (E.a[g].C + nc - 1) / nc;                                  // column blocks per member
+    E.off[g] = total;
+    total += (E.nblk[g] * E.a[g].nm + sub - 1) / sub;
+  }
+  E.off[E.ng] = total;
+  if (fp32) ens_column_pass_kernel<float><<<total, ENS_COL_NTH, 0, st>>>(E);
+  else ens_column_pass_kernel<double><<<total, ENS_COL_NTH, 0, st>>>(E);
   return 0;
 }
 

@@ -819,6 +819,205 @@ extern "C" int dns_rk3_step_graph(dns_handle *h, int nsteps) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Ensemble of several sizes: the handles (one per size, each with its same-size members) advance
+// together with one grouped launch per kernel class per substep (ensemble.hip).
+struct dns_ensemble {
+  std::vector<dns_handle *> h;
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t done = nullptr;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  double tsum[7] = {};
+  int tcount = 0;
+};
+
+static void ens_mark(dns_ensemble *e, int q) {
+  if (e->timing) (void)hipEventRecord(e->ev[q], e->stream);
+}
+
+static int ens_substep(dns_ensemble *e, int kRK) {
+  const int ng = (int)e->h.size();
+  dnsk::EnsSmall S;
+  memset(&S, 0, sizeof S);
+  dnsk::EnsPlaneLaunch P;
+  memset(&P, 0, sizeof P);
+  dnsk::EnsColLaunch Cl;
+  memset(&Cl, 0, sizeof Cl);
+  int flags[dnsk::ENS_MAXG] = {0, 0, 0, 0};
+  S.ng = P.ng = Cl.ng = ng;
+  for (int g = 0; g < ng; g++) {
+    dns_handle *h = e->h[g];
+    S.st[g] = h->st;
+    S.slots[g] = h->cfls;
+    S.u1[g] = h->u1;
+    S.u3[g] = h->u3;
+    S.ms[g] = dnsk::member_strides(h->C, h->nyu, h->nyv).ru;
+    S.nm[g] = h->nm;
+    S.C[g] = h->C;
+    P.a[g] = plane_args(h, kRK);
+    P.a[g].j0 = 1;
+    P.a[g].nbj = h->nyu;
+    Cl.a[g] = col_args(h, kRK);
+    flags[g] = h->prm.flag_ctpress == 0;
+  }
+  ens_mark(e, 0);
+  dnsk::launch_ens_reset(S, kRK, e->stream);
+  if (kRK == 1) dnsk::launch_ens_ghost(S, e->stream);
+  int rc = dnsk::launch_ens_plane(P, e->fp32, 0, e->stream);
+  ens_mark(e, 1);
+  rc |= dnsk::launch_ens_plane(P, e->fp32, 1, e->stream);
+  rc |= dnsk::launch_ens_plane(P, e->fp32, 2, e->stream);
+  ens_mark(e, 2);
+  rc |= dnsk::launch_ens_plane(P, e->fp32, 3, e->stream);
+  rc |= dnsk::launch_ens_plane(P, e->fp32, 4, e->stream);
+  ens_mark(e, 3);
+  if (kRK == 1) dnsk::launch_ens_cfl_reduce(S, e->stream);
+  rc |= dnsk::launch_ens_column_pass(Cl, e->fp32, e->stream);
+  dnsk::launch_ens_meanflow_post(Cl, flags, e->stream);
+  ens_mark(e, 4);
+  if (rc) return fail(DNS_ERR_ARG, "ensemble: a size is not instantiated for grouped launches");
+  HIPCHK(hipGetLastError());
+  return DNS_OK;
+}
+
+static void ens_accumulate_timing(dns_ensemble *e) {
+  float ms[5];
+  (void)hipEventSynchronize(e->ev[4]);
+  for (int q = 0; q < 4; q++) (void)hipEventElapsedTime(&ms[q], e->ev[q], e->ev[q + 1]);
+  (void)hipEventElapsedTime(&ms[4], e->ev[0], e->ev[4]);
+  e->tsum[0] += ms[0]; e->tsum[1] += ms[1]; e->tsum[2] += ms[2]; e->tsum[3] += ms[3]; e->tsum[6] += ms[4];
+  e->tcount++;
+}
+
+static void ens_destroy_all(dns_ensemble *e) {
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->done) (void)hipEventDestroy(e->done);
+  for (int q = 0; q < 5; q++)
+    if (e->ev[q]) (void)hipEventDestroy(e->ev[q]);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" int dns_ensemble_create(dns_handle *const *hs, int n, dns_ensemble **out) {
+  if (!hs || !out) return fail(DNS_ERR_ARG, "dns_ensemble_create: null argument");
+  *out = nullptr;
+  if (n < 1 || n > dnsk::ENS_MAXG) return fail(DNS_ERR_ARG, "dns_ensemble_create: %d groups (1..%d)", n, dnsk::ENS_MAXG);
+  for (int g = 0; g < n; g++) {
+    const dns_handle *h = hs[g];
+    if (!h) return fail(DNS_ERR_ARG, "dns_ensemble_create: handle %d is null", g);
+    for (int q = 0; q < g; q++)
+      if (hs[q] == h) return fail(DNS_ERR_ARG, "dns_ensemble_create: handle %d appears twice", g);
+    if (h->fp32 != hs[0]->fp32) return fail(DNS_ERR_ARG, "dns_ensemble_create: the groups' precisions differ");
+    if (!dnsk::ens_plane_length_supported(h->Mx) || !dnsk::ens_plane_length_supported(h->Mz) ||
+        !dnsk::ens_plane_rows_ok(h->Mx, h->Mz))
+      return fail(DNS_ERR_ARG, "dns_ensemble_create: group %d has %d x %d physical points; grouped launches take "
+                  "48, 96 or 192 in each direction", g, h->Mx, h->Mz);
+    if (!dnsk::ens_column_rows_supported(h->nyu))
+      return fail(DNS_ERR_ARG, "dns_ensemble_create: group %d has ny = %d; grouped launches take 9 <= ny <= 161", g,
+                  h->ny);
+    if (h->bj < h->nyu)
+      return fail(DNS_ERR_ARG, "dns_ensemble_create: group %d splits its planes into batches (plane_batch)", g);
+  }
+  dns_ensemble *e = new dns_ensemble();
+  e->h.assign(hs, hs + n);
+  e->fp32 = hs[0]->fp32;
+  hipError_t r = hipStreamCreate(&e->stream);
+  if (r == hipSuccess) r = hipEventCreateWithFlags(&e->done, hipEventDisableTiming);
+  for (int q = 0; q < 5 && r == hipSuccess; q++) r = hipEventCreate(&e->ev[q]);
+  if (r != hipSuccess) {
+    ens_destroy_all(e);
+    return fail(DNS_ERR_HIP, "dns_ensemble_create: %s", hipGetErrorString(r));
+  }
+  *out = e;
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_destroy(dns_ensemble *e) {
+  if (!e) return DNS_OK;
+  ens_destroy_all(e);
+  return DNS_OK;
+}
+
+// Work already queued on the handles' streams (uploads) comes first; the handles' later calls
+// (downloads, their own steps) wait for the ensemble's steps.
+static int ens_begin(dns_ensemble *e, const char *who) {
+  if (!e) return fail(DNS_ERR_ARG, "%s: null ensemble", who);
+  for (dns_handle *h : e->h) {
+    for (int m = 0; m < h->nm; m++)
+      if (!h->uploaded[m]) return fail(DNS_ERR_STATE, "%s: no fields uploaded for a member", who);
+    HIPCHK(hipEventRecord(e->done, h->stream));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->done, 0));
+  }
+  return DNS_OK;
+}
+static int ens_end(dns_ensemble *e) {
+  HIPCHK(hipEventRecord(e->done, e->stream));
+  for (dns_handle *h : e->h) HIPCHK(hipStreamWaitEvent(h->stream, e->done, 0));
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_step(dns_ensemble *e, int nsteps) {
+  int rc = ens_begin(e, "dns_ensemble_step");
+  if (rc) return rc;
+  for (int s = 0; s < nsteps; s++)
+    for (int kRK = 1; kRK <= 3; kRK++) {
+      if ((rc = ens_substep(e, kRK))) return rc;
+      if (e->timing) ens_accumulate_timing(e);
+    }
+  return ens_end(e);
+}
+
+extern "C" int dns_ensemble_step_graph(dns_ensemble *e, int nsteps) {
+  int rc = ens_begin(e, "dns_ensemble_step_graph");
+  if (rc) return rc;
+  if (!e->graph) {
+    const bool tm = e->timing;
+    e->timing = false;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int kRK = 1; kRK <= 3 && rc == 0; kRK++) rc = ens_substep(e, kRK);
+    hipError_t r = hipStreamEndCapture(e->stream, &g);
+    e->timing = tm;
+    if (rc || r != hipSuccess) {
+      if (r == hipSuccess && g) (void)hipGraphDestroy(g);
+      if (rc) return rc;
+      return fail(DNS_ERR_HIP, "ensemble graph capture: %s", hipGetErrorString(r));
+    }
+    hipGraphExec_t x = nullptr;
+    r = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) return fail(DNS_ERR_HIP, "ensemble graph instantiate: %s", hipGetErrorString(r));
+    e->graph = x;
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(e->graph, e->stream));
+  return ens_end(e);
+}
+
+extern "C" int dns_ensemble_sync(dns_ensemble *e) {
+  if (!e) return fail(DNS_ERR_ARG, "dns_ensemble_sync: null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_set_timing(dns_ensemble *e, int on) {
+  if (!e) return fail(DNS_ERR_ARG, "dns_ensemble_set_timing: null ensemble");
+  e->timing = on != 0;
+  memset(e->tsum, 0, sizeof e->tsum);
+  e->tcount = 0;
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_get_timing(dns_ensemble *e, double *t, int *n) {
+  if (!e || !t) return fail(DNS_ERR_ARG, "dns_ensemble_get_timing: null argument");
+  for (int q = 0; q < 7; q++) t[q] = e->tcount ? e->tsum[q] / e->tcount : 0.0;
+  if (n) *n = e->tcount;
+  return DNS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Kernel variants.  Every variant of a kernel gives bit-identical results: the plane kernels' variants
 // differ in tile shape (same butterflies, same order of operations per element), the column sweep's in
 // how the three solves map to wavefronts (same arithmetic per column).

@@ -524,6 +524,21 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
+// block_max over the NTH threads tid = 0..NTH-1 of one body of a workgroup that may run several bodies
+// side by side (each with its own red); every thread of the workgroup must call it.  Valid on tid 0.
+template <int NTH>
+__device__ __forceinline__ double block_max_sub(double v, double *red, int tid) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int w = tid >> 6, l = tid & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < NTH / 64; q++) v = fmax(v, red[q]);
+  }
+  return v;
+}
+
 template <int NTH>
 __device__ __forceinline__ double block_max(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));

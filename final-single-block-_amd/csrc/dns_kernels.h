@@ -212,6 +212,43 @@ int four_to_phys_scratch(void **ptr);
 // launch_meanflow_post)
 bool column_pass_supported(int nyu);
 int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered
+// ---- ensemble of several sizes: grouped launches (ensemble.hip, column_pass.hip) ----
+constexpr int ENS_MAXG = 4;          // size groups per ensemble
+constexpr int ENS_PLANE_NTH = 128;   // block size of the grouped plane launches
+// one grouped plane launch: group g's arguments, the transform length of this kernel class, its grid
+// (gx, gy) and its first block (off); filled by launch_ens_plane
+struct EnsPlaneLaunch {
+  PlaneArgs a[ENS_MAXG];
+  int len[ENS_MAXG], gx[ENS_MAXG], gy[ENS_MAXG], off[ENS_MAXG + 1];
+  int ng;
+};
+// the small per-substep kernels of every group
+struct EnsSmall {
+  DnsState *st[ENS_MAXG];
+  double *slots[ENS_MAXG];
+  const double2 *u1[ENS_MAXG], *u3[ENS_MAXG];
+  size_t ms[ENS_MAXG];               // member stride of u1/u3
+  int nm[ENS_MAXG], C[ENS_MAXG], gx[ENS_MAXG], off[ENS_MAXG + 1];
+  int ng;
+};
+// one grouped column-pass launch (shape by row count: rows <= 96 or <= 160, 512 threads)
+struct EnsColLaunch {
+  ColArgs a[ENS_MAXG];
+  int nblk[ENS_MAXG], off[ENS_MAXG + 1];
+  int ng;
+};
+bool ens_plane_length_supported(int n);
+int ens_plane_rows_ok(int Mx, int Mz);
+// which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; every group's j0 = 1, nbj = nyu
+int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, hipStream_t st);
+void launch_ens_reset(const EnsSmall &E, int kRK, hipStream_t st);
+void launch_ens_ghost(EnsSmall E, hipStream_t st);
+void launch_ens_cfl_reduce(const EnsSmall &E, hipStream_t st);
+bool ens_column_rows_supported(int nyu);
+int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st);
+// meanflow_ctU after the column pass for the groups with a constant flow rate (flags[g] = 1)
+void launch_ens_meanflow_post(const EnsColLaunch &E, const int *flags, hipStream_t st);
+
 // layout
 void launch_ref_to_dev(const double2 *ref, double2 *dev, int C, int nrows, hipStream_t st);
 void launch_dev_to_ref(const double2 *dev, double2 *ref, int C, int nrows, hipStream_t st);

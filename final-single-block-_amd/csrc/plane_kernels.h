@@ -29,16 +29,23 @@ template <class PL, int N> constexpr int PL_M = N / plan_info<PL>::first;
 // Grid: (column tiles, field, chunk of ZCH planes); ZCH + 1 field rows per workgroup.  Field rows walked: u1, u3 rows jc0..jc1+1 (plain
 // -> u-plane r, interpolated with the row below -> v-plane r-1); u2 rows jc0-1..jc1 (plain -> v-plane r,
 // interpolated with the row below -> u-plane r).
-template <typename R, int MZ, int T, int NTH, class PL, int ZCH, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
+// Every plane kernel is a body (a struct with the LDS it needs and run()) that takes its logical block
+// index and an LDS buffer, and a thin __global__ wrapper; the ensemble's grouped launches
+// (ensemble.hip) run the bodies of several transform lengths in one launch.
+template <typename R, int MZ, int T, int NTH_, class PL, int ZCH, int PS = 4>
+struct ZinvBody {
+  static constexpr int NTH = NTH_;
   using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
-  __shared__ cx<R> s[L::SIZE];
+  static constexpr size_t LDS = sizeof(cx<R>) * L::SIZE;
+  // grid (gx column tiles, gy = 3 fields, chunks x members); lb = logical block, tile fastest
+  __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
+};
+template <typename R, int MZ, int T, int NTH_, class PL, int ZCH, int PS>
+__device__ __forceinline__ void ZinvBody<R, MZ, T, NTH_, PL, ZCH, PS>::run(const PlaneArgs &a_, int lb, int gx,
+                                                                           int gy, unsigned char *lds) {
+  cx<R> *s = reinterpret_cast<cx<R> *>(lds);
   constexpr int KS = last_slots<L, NTH, PL>();
   const int nch = (a_.nbj + ZCH - 1) / ZCH;  // grid z = chunks x ensemble members
-  // logical block (tile fastest, then field, then chunk), XCD-aware: neighbouring tiles of one row run
-  // on one XCD and share the lines at their edges (a k-row of Kx modes is not line-aligned)
-  const int gx = gridDim.x, gy = gridDim.y;
-  const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
   const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
   const PlaneArgs a = plane_view<R>(a_, bz / nch);
   const int f = by;
@@ -100,6 +107,17 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
   }
 }
 
+template <typename R, int MZ, int T, int NTH, class PL, int ZCH, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
+  using B = ZinvBody<R, MZ, T, NTH, PL, ZCH, PS>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  // logical block (tile fastest, then field, then chunk), XCD-aware: neighbouring tiles of one row run
+  // on one XCD and share the lines at their edges (a k-row of Kx modes is not line-aligned)
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  B::run(a, lb, gx, gy, lds);
+}
+
 // ------------------------------------------------------------------------------------------------
 // x-pass.  A real row x of length MX is packed as z_m = x_{2m} + i x_{2m+1} (H = MX/2 complex), so one
 // length-H complex FFT does each real transform.  Inverse (rft_buff.f:76-84 semantics: Im c_0 and
@@ -113,23 +131,34 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a_) {
 
 template <bool VPL> constexpr int xpass_slots() { return VPL ? 3 : 4; }  // LDS slots per row
 
-template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
-  const PlaneArgs a = plane_view<R>(a_, blockIdx.y / a_.nbj);  // grid y = planes x ensemble members
-  constexpr int H = MX / 2;
-  constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
-  constexpr int NS = RB * NSL;
+template <typename R, int MX, int RB, int NTH_, bool VPL, class PLI, class PLF, int PS = 4>
+struct XpassBody {
+  static constexpr int NTH = NTH_;
+  static constexpr int H = MX / 2;
+  static constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
+  static constexpr int NS = RB * NSL;
   // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers; twiddles from
   // the pass-blocked copy after twh (contiguous loads across the butterflies of a pass)
   using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false, true>;  // inverse: active f*RB + r -> slot r*NSL + f
   using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false, true>;  // forward: slots r*NSL + p, p < NP
+  static constexpr size_t LDS = sizeof(cx<R>) * LI::SIZE + sizeof(double) * (NTH / 64);
+  // grid (gx row tiles, planes x members); lb = bx + gx * by
+  __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
+};
+template <typename R, int MX, int RB, int NTH_, bool VPL, class PLI, class PLF, int PS>
+__device__ __forceinline__ void XpassBody<R, MX, RB, NTH_, VPL, PLI, PLF, PS>::run(const PlaneArgs &a_, int lb,
+                                                                                  int gx, int gy,
+                                                                                  unsigned char *lds) {
+  (void)gy;
+  const int bx = lb % gx, by = lb / gx;
+  const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
   static_assert(std::is_same<PLI, typename XPlanT<H>::type>::value &&
                     std::is_same<PLF, typename XPlanT<H>::type>::value,
                 "twh's pass-blocked twiddles are built for the plan XPlanT<H>");
-  __shared__ cx<R> A[LI::SIZE];
-  __shared__ double red[NTH / 64];
-  const int jj = blockIdx.y % a.nbj, j = a.j0 + jj;
-  const int z0 = blockIdx.x * RB;
+  cx<R> *A = reinterpret_cast<cx<R> *>(lds);
+  double *red = reinterpret_cast<double *>(lds + sizeof(cx<R>) * LI::SIZE);
+  const int jj = by % a.nbj, j = a.j0 + jj;
+  const int z0 = bx * RB;
   constexpr int Kx = MX / 3 + 1;   // retained x-modes Nx/2 + 1, Nx = 2 MX/3 (the launcher checks a.Kx)
   const int MZ = a.Mz;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
@@ -184,7 +213,7 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
     const double b1 = block_max<NTH>((double)m1, red);
     const double b3 = block_max<NTH>((double)m3, red);
     if (threadIdx.x == 0) {  // spread over CFL_SLOTS addresses: one address for every block serialises
-      const int slot = (blockIdx.x + blockIdx.y) & (CFL_SLOTS - 1);
+      const int slot = (bx + by) & (CFL_SLOTS - 1);
       atomic_max_nonneg(&a.cfl[slot], b1);
       atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);
     }
@@ -213,6 +242,13 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
   }
 }
 
+template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
+  using B = XpassBody<R, MX, RB, NTH, VPL, PLI, PLF, PS>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  B::run(a, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, lds);
+}
+
 // ------------------------------------------------------------------------------------------------
 // The product fields are transformed one at a time, ordered so that at most two outputs are live in
 // registers: u-planes uw (starts der_z(uw) and der_x(uw)), uu (completes Nxz1 = der_x(uu) + der_z(uw)),
@@ -220,15 +256,21 @@ __global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a_) {
 // wv (flux, completes Nxz2 = der_x(uv) + der_z(wv)).  Sums are formed in the reference's operand order
 // (ops_in_planes2, FOU3D.f90:1590-1640).  The kz = -Nz/2 row is inside der_x/der_z's zero band; the raw
 // fluxes keep it.
-template <typename R, int MZ, int T, int NTH, bool VPL, class PL, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a_) {
-  // logical block (tile fastest), XCD-aware: neighbouring tiles share the output lines at their edges
-  const int gx = gridDim.x;
-  const int lb = xcd_block(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+template <typename R, int MZ, int T, int NTH_, bool VPL, class PL, int PS = 4>
+struct ZfwdBody {
+  static constexpr int NTH = NTH_;
+  using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
+  static constexpr size_t LDS = sizeof(cx<R>) * L::SIZE;
+  // grid (gx column tiles, planes x members); lb = logical block, tile fastest
+  __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
+};
+template <typename R, int MZ, int T, int NTH_, bool VPL, class PL, int PS>
+__device__ __forceinline__ void ZfwdBody<R, MZ, T, NTH_, VPL, PL, PS>::run(const PlaneArgs &a_, int lb, int gx,
+                                                                           int gy, unsigned char *lds) {
+  (void)gy;
   const int bx = lb % gx, by = lb / gx;
   const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
-  using L = Lay<MZ, T, T, 1, 1, PS, false>;  // T adjacent columns, column fastest across threads
-  __shared__ cx<R> s[L::SIZE];
+  cx<R> *s = reinterpret_cast<cx<R> *>(lds);
   constexpr int PERMAX = ((2 * MZ / 3) * T + NTH - 1) / NTH;
   constexpr int NP = VPL ? 2 : 4;
   const int jj = by % a.nbj, j = a.j0 + jj;
@@ -283,6 +325,16 @@ __global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a_) {
     }
     __syncthreads();
   }
+}
+
+template <typename R, int MZ, int T, int NTH, bool VPL, class PL, int WPE = 1, int PS = 4>
+__global__ void __launch_bounds__(NTH, WPE) zfwd_kernel(PlaneArgs a) {
+  using B = ZfwdBody<R, MZ, T, NTH, VPL, PL, PS>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  // logical block (tile fastest), XCD-aware: neighbouring tiles share the output lines at their edges
+  const int gx = gridDim.x;
+  const int lb = xcd_block(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+  B::run(a, lb, gx, gridDim.y, lds);
 }
 
 }  // namespace dnsk

@@ -46,7 +46,9 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
            "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
            "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats",
-           "dns_four_to_phys", "dns_four_to_phys_noalloc"]
+           "dns_four_to_phys", "dns_four_to_phys_noalloc", "dns_ensemble_create", "dns_ensemble_destroy",
+           "dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_sync", "dns_ensemble_set_timing",
+           "dns_ensemble_get_timing"]
 
 
 def lib():
@@ -96,6 +98,12 @@ def lib():
     L.dns_get_stats.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.dns_four_to_phys.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp]
     L.dns_four_to_phys_noalloc.argtypes = [ctypes.c_int] * 5 + [vp, vp, vp]
+    L.dns_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    for n in ("dns_ensemble_destroy", "dns_ensemble_sync"):
+        getattr(L, n).argtypes = [vp]
+    for n in ("dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_set_timing"):
+        getattr(L, n).argtypes = [vp, ctypes.c_int]
+    L.dns_ensemble_get_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -345,3 +353,38 @@ class DNS:
 
     def planes_to_modes(self, d_planes, nrows, d_cols):
         _chk(lib().dns_planes_to_modes(self.h, d_planes, nrows, d_cols))
+
+
+class Ensemble:
+    """Several DNS handles (one per size, each with its members) advanced together: one grouped launch per
+    kernel class per RK3 substep for every member (dns_ensemble_*).  The handles stay usable for uploads,
+    downloads and scalars, and must stay open while the ensemble is."""
+
+    def __init__(self, handles):
+        self.handles = list(handles)
+        arr = (ctypes.c_void_p * len(self.handles))(*[h.h.value for h in self.handles])
+        e = ctypes.c_void_p()
+        _chk(lib().dns_ensemble_create(arr, len(self.handles), ctypes.byref(e)))
+        self.e = e
+
+    def step(self, n=1): _chk(lib().dns_ensemble_step(self.e, n))
+    def step_graph(self, n=1): _chk(lib().dns_ensemble_step_graph(self.e, n))
+    def sync(self): _chk(lib().dns_ensemble_sync(self.e))
+    def set_timing(self, on): _chk(lib().dns_ensemble_set_timing(self.e, 1 if on else 0))
+
+    def timing(self):
+        t = np.zeros(7)
+        n = ctypes.c_int()
+        _chk(lib().dns_ensemble_get_timing(self.e, _p(t), ctypes.byref(n)))
+        return dict(zinv=t[0], xpass=t[1], zfwd=t[2], col_pass=t[3], substep=t[6], n=n.value)
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().dns_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -195,6 +195,42 @@ module dns_hip
       real(c_double), intent(out) :: fout(*)
       real(c_double), intent(out) :: ms
     end function
+    ! ensemble of several sizes: handles(1:n) are type(c_ptr) handles from dns_create
+    integer(c_int) function dns_ensemble_create(handles, n, e) bind(C, name='dns_ensemble_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: handles(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function dns_ensemble_destroy(e) bind(C, name='dns_ensemble_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function dns_ensemble_step(e, nsteps) bind(C, name='dns_ensemble_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function dns_ensemble_step_graph(e, nsteps) bind(C, name='dns_ensemble_step_graph')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function dns_ensemble_sync(e) bind(C, name='dns_ensemble_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function dns_ensemble_set_timing(e, on) bind(C, name='dns_ensemble_set_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function dns_ensemble_get_timing(e, t, n) bind(C, name='dns_ensemble_get_timing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), intent(out) :: t(7)
+      integer(c_int), intent(out) :: n
+    end function
     ! device pointers (type(c_ptr)) and a stream (c_null_ptr: the null stream); no device allocation
     integer(c_int) function dns_four_to_phys_noalloc(Nx, Nz, Mx, Mz, nplanes, d_in, d_out, stream) &
         bind(C, name='dns_four_to_phys_noalloc')

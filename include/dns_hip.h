@@ -189,6 +189,29 @@ int dns_four_to_phys_noalloc(int Nx, int Nz, int Mx, int Mz, int nplanes, const 
 int dns_set_column_path(dns_handle *h, int path);
 int dns_get_column_path(dns_handle *h, int *path);
 
+/* ---- ensemble of several sizes ----
+ * No counterpart in the reference, which runs one channel per job; every member runs the reference's
+ * single-channel RK3 loop (littleharsh.f90:163-233).  An ensemble joins 1..4 handles (one per size, each
+ * with its same-size members, dns_params.members) so that all their members advance with one grouped launch
+ * per kernel class per RK3 substep (z-inverse, x-pass u/v, z-forward u/v, column stage, and the small CFL
+ * and reset kernels), on the ensemble's own stream.  Sizes: Mx and Mz each 48, 96 or 192, 9 <= ny <= 161,
+ * one precision for all, no plane batches; others are rejected with DNS_ERR_ARG.  The handles stay owned
+ * by the caller and must outlive the ensemble; their uploads, downloads and scalars work as before (work on
+ * a handle's stream is ordered before and after the ensemble's steps).  A member's result equals the same
+ * channel stepped alone, bit for bit. */
+typedef struct dns_ensemble dns_ensemble;
+int dns_ensemble_create(dns_handle *const *handles, int n, dns_ensemble **out);
+int dns_ensemble_destroy(dns_ensemble *e);
+/* nsteps RK3 steps of every member (iter += 1 per step, as dns_rk3_step) */
+int dns_ensemble_step(dns_ensemble *e, int nsteps);
+/* One RK3 step captured once in a hipGraph on the ensemble's stream and replayed nsteps times. */
+int dns_ensemble_step_graph(dns_ensemble *e, int nsteps);
+int dns_ensemble_sync(dns_ensemble *e);
+/* Event timing of dns_ensemble_step as dns_set_timing/dns_get_timing: t[0] resets + z-inverse, t[1] x-pass,
+ * t[2] z-forward, t[3] column stage (+ CFL reduction, mean-flow correction), t[6] whole substep (ms). */
+int dns_ensemble_set_timing(dns_ensemble *e, int on);
+int dns_ensemble_get_timing(dns_ensemble *e, double *t, int *n);
+
 /* Waits for all work on the handle's stream. */
 int dns_sync(dns_handle *h);
 /* Average device duration (ms) per substep of each kernel, from HIP events on the handle's stream,

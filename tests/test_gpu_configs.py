@@ -4,8 +4,8 @@
 * 32x32x153 at fixed dt = 1e-3, 10 steps (the reference's CPU-runnable case as a fixed-step run);
 * the reference's own 32x32x153 initial fields (input/*_0032x0032x0153_t00000.dat, as a fixture);
 * 128x128x153 over 1000 CFL steps against the reference's own trajectory, step by step through dt;
-* the 48-member ensemble of three sizes (24 x 32^2x153, 16 x 64^2x97, 8 x 128^2x153), FP64 and mixed,
-  autotuned, replayed from hipGraphs;
+* the 48-member ensemble of three sizes (24 x 32^2x153, 16 x 64^2x97, 8 x 128^2x153), FP64 and mixed, one
+  grouped launch per kernel class per substep, replayed from a hipGraph, against single channels;
 * one full-size FP64 step at 384x384x257 and at 768x384x385 against the oracle, with the mixed path's
   error against the oracle.
 FP64 bar: relative L2 <= 1e-12.
@@ -164,10 +164,10 @@ def ensemble_members():
 
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
 def test_ensemble_48_members_three_sizes(prec):
-    """The 48-member ensemble, autotuned, each group's RK3 step replayed from its captured hipGraph (3 steps):
-    every member equals the same channel run alone bit for bit (own IC, Re, mpgx, CFL dt), and the first
-    member of each size agrees with the oracle (FP64: <= 1e-12; mixed: its FP32 plane stage's error,
-    <= 1e-6 after 3 steps)."""
+    """The 48-member ensemble of three sizes in one grouped launch per kernel class per substep, its RK3 step
+    captured once in a hipGraph and replayed (3 steps): every member equals the same channel run alone
+    (autotuned) bit for bit (own IC, Re, mpgx, CFL dt), and the first member of each size agrees with the
+    oracle (FP64: <= 1e-12; mixed: its FP32 plane stage's error, <= 1e-6 after 3 steps)."""
     nsteps = 3
     members = ensemble_members()
     handles, ics = [], []
@@ -180,19 +180,21 @@ def test_ensemble_48_members_three_sizes(prec):
             e.set_member_params(m, Re, mp)
             e.upload(*st, member=m)
             sts.append(st)
-        e.autotune(iters=2)
         handles.append(e)
         ics.append(sts)
-    for e in handles:          # one stream per handle: the three groups run concurrently
-        e.step_graph(nsteps)
-    for e in handles:
-        e.sync()
+    ens = dnship.Ensemble(handles)
+    ens.step_graph(nsteps)
+    ens.sync()
+    ens.close()
     for (nm, Mx, Mz, ny), grp, sts, e in zip(ENSEMBLE, members, ics, handles):
         print("case: ensemble group %dx%dx%d members vs single channels" % (Mx, Mz, ny), flush=True)
         for m, ((Re, mp, _), st) in enumerate(zip(grp, sts)):
             got = e.download(member=m)
             s = dnship.DNS(Mx, Mz, ny, Re=Re, mpgx=mp, precision=prec)
             s.upload(*st)
+            if m == 0:
+                s.autotune(iters=2)
+                s.upload(*st)
             s.step(nsteps)
             ref = s.download()
             sref = s.scalars()
@@ -214,6 +216,63 @@ def test_ensemble_48_members_three_sizes(prec):
             elif k != "p":
                 assert err <= 1e-6, k
         e.close()
+
+
+@pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
+def test_ensemble_grouped_eager_and_graph(prec):
+    """Grouped launches with a constant-flow-rate group (mean-flow correction after the column pass), a
+    rectangular plane and a group of one: 2 eager steps then 2 graph-replayed steps equal every member's
+    single-channel run bit for bit, t, dt, mpgx and iter included; the handles' own downloads wait for the
+    ensemble's work."""
+    groups = [(48, 48, 33, 0, [(3250.0, -3.0675e-3, 31), (2500.0, -3.5e-3, 32)]),
+              (96, 192, 65, 1, [(4000.0, -2.8e-3, 33)]),
+              (192, 96, 17, 1, [(3000.0, -3.2e-3, 34), (3500.0, -3.0e-3, 35), (2800.0, -3.3e-3, 36)])]
+    handles, ics = [], []
+    for Mx, Mz, ny, ctp, mem in groups:
+        h = dnship.DNS(Mx, Mz, ny, precision=prec, members=len(mem), flag_ctpress=ctp)
+        sts = []
+        for m, (Re, mp, sd) in enumerate(mem):
+            st = dns_ic.seeded_channel(Mx, Mz, ny, seed=sd, broadband=True)
+            h.set_member_params(m, Re, mp)
+            h.upload(*st, member=m)
+            sts.append(st)
+        handles.append(h)
+        ics.append(sts)
+    print("case: grouped ensemble, 3 groups, prec %d" % prec, flush=True)
+    ens = dnship.Ensemble(handles)
+    ens.step(2)
+    ens.step_graph(2)
+    for (Mx, Mz, ny, ctp, mem), sts, h in zip(groups, ics, handles):
+        for m, ((Re, mp, _), st) in enumerate(zip(mem, sts)):
+            got, sc = h.download(member=m), h.scalars(member=m)
+            s = dnship.DNS(Mx, Mz, ny, Re=Re, mpgx=mp, precision=prec, flag_ctpress=ctp)
+            s.upload(*st)
+            s.step(4)
+            ref, sref = s.download(), s.scalars()
+            s.close()
+            for x, y in zip(got, ref):
+                assert np.array_equal(x, y), (Mx, Mz, ny, m)
+            for k in ("t", "dt", "mpgx", "dgx", "Qx", "iter", "divmax"):
+                assert sc[k] == sref[k], (Mx, Mz, ny, m, k)
+    ens.close()
+    for h in handles:
+        h.close()
+
+
+def test_ensemble_rejects_unsupported_groups():
+    a = dnship.DNS(48, 48, 33)
+    b = dnship.DNS(576, 576, 17)                  # 576 points: not a grouped-launch length
+    c = dnship.DNS(48, 48, 193)                   # 192 rows: beyond the grouped column pass
+    d = dnship.DNS(48, 48, 33, precision=dnship.PREC_MIXED)
+    for bad in ([a, b], [a, c], [a, d], [a, a], []):
+        with pytest.raises(dnship.DnsError):
+            dnship.Ensemble(bad)
+    e = dnship.Ensemble([a])
+    with pytest.raises(dnship.DnsError):          # no fields uploaded
+        e.step(1)
+    e.close()
+    for h in (a, b, c, d):
+        h.close()
 
 
 @pytest.mark.parametrize("Mx,Mz,ny,Re,retau,seed", [
