@@ -540,17 +540,34 @@ static void launch_pass(const ColArgs &a, hipStream_t st) {
   else column_pass_kernel<NC, P, M, double><<<grid, NC * P, 0, st>>>(a);
 }
 
-int launch_column_pass(const ColArgs &a, hipStream_t st) {
+// Shapes of a bucket for the autotuner: the same P and M (so the same arithmetic per column, the same
+// bits) with NC = the default, then the other two columns-per-workgroup counts that keep NC * P within
+// 64..512 threads.
+int column_pass_variants(int nyu) {
+  switch (column_pass_rows(nyu)) {
+    case 32: case 64: case 96: case 128: case 160: case 384: return 3;
+    case 192: case 256: case 512: return 2;
+    default: return 0;
+  }
+}
+
+int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
   switch (column_pass_rows(a.nyu)) {
-    case 32: launch_pass<8, 8, 4>(a, st); break;
-    case 64: launch_pass<8, 16, 4>(a, st); break;
-    case 96: launch_pass<8, 32, 3>(a, st); break;
-    case 128: launch_pass<8, 32, 4>(a, st); break;
-    case 160: launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
-    case 192: launch_pass<8, 64, 3>(a, st); break;
-    case 256: launch_pass<8, 64, 4>(a, st); break;
-    case 384: launch_pass<8, 64, 6>(a, st); break;
-    case 512: launch_pass<4, 64, 8>(a, st); break;
+    case 32: if (v == 1) launch_pass<16, 8, 4>(a, st); else if (v == 2) launch_pass<32, 8, 4>(a, st);
+             else launch_pass<8, 8, 4>(a, st); break;
+    case 64: if (v == 1) launch_pass<16, 16, 4>(a, st); else if (v == 2) launch_pass<4, 16, 4>(a, st);
+             else launch_pass<8, 16, 4>(a, st); break;
+    case 96: if (v == 1) launch_pass<16, 32, 3>(a, st); else if (v == 2) launch_pass<4, 32, 3>(a, st);
+             else launch_pass<8, 32, 3>(a, st); break;
+    case 128: if (v == 1) launch_pass<16, 32, 4>(a, st); else if (v == 2) launch_pass<4, 32, 4>(a, st);
+              else launch_pass<8, 32, 4>(a, st); break;
+    case 160: if (v == 1) launch_pass<8, 32, 5>(a, st); else if (v == 2) launch_pass<4, 32, 5>(a, st);
+              else launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
+    case 192: if (v == 1) launch_pass<4, 64, 3>(a, st); else launch_pass<8, 64, 3>(a, st); break;
+    case 256: if (v == 1) launch_pass<4, 64, 4>(a, st); else launch_pass<8, 64, 4>(a, st); break;
+    case 384: if (v == 1) launch_pass<4, 64, 6>(a, st); else if (v == 2) launch_pass<2, 64, 6>(a, st);
+              else launch_pass<8, 64, 6>(a, st); break;
+    case 512: if (v == 1) launch_pass<2, 64, 8>(a, st); else launch_pass<4, 64, 8>(a, st); break;
     default: return 1;
   }
   return 0;

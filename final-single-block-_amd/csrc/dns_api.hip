@@ -43,8 +43,8 @@ struct dns_handle {
   int Mx, Mz, Nx, Nz, Kx, C, ny, nyu, nyv;
   bool fp32;
   int bj;  // plane index j per plane-stage group (u- and v-planes of each j)
-  // variant per kernel: z-inverse, x-pass u/v, z-forward u/v, column sweeps up/down/final
-  int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // variant per kernel: z-inverse, x-pass u/v, z-forward u/v, column sweeps up/down/final, one-pass column
+  int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool colpass = false;  // column stage: one-pass kernel (true) or the three sweeps
   double *stat_acc = nullptr, *stat_sp = nullptr, *stat_part = nullptr;  // statistics (allocated on first use)
   int istat = 0;
@@ -744,7 +744,8 @@ static int fused_substep(dns_handle *h, int kRK) {
   // the plane stage's reset has cleared it
   if (kRK != 1) dnsk::launch_divmax_reset(h->st, h->nm, h->stream);
   if (h->colpass) {
-    if (dnsk::launch_column_pass(a, h->stream)) return fail(DNS_ERR_ARG, "column pass: %d rows not instantiated", h->nyu);
+    if (dnsk::launch_column_pass(a, h->var[8], h->stream))
+      return fail(DNS_ERR_ARG, "column pass: %d rows not instantiated", h->nyu);
     // constant flow rate: meanflow_ctU on mode (0,0), which commutes with solveP and v_corr there
     if (h->prm.flag_ctpress == 0 && dnsk::launch_meanflow_post(a, h->stream))
       return fail(DNS_ERR_ARG, "column pass: %d rows exceed the mean-flow kernel", h->nyu);
@@ -1024,6 +1025,7 @@ extern "C" int dns_ensemble_get_timing(dns_ensemble *e, double *t, int *n) {
 static int kern_len(const dns_handle *h, int c) { return (c == 1 || c == 2) ? h->Mx : h->Mz; }
 static int kern_variants(const dns_handle *h, int c) {
   if (c < 5) return dnsk::plane_variants(kern_len(h, c));
+  if (c == 8) return dnsk::column_pass_supported(h->nyu) ? dnsk::column_pass_variants(h->nyu) : 1;
   return 3;
 }
 
@@ -1098,9 +1100,12 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
     double tbest = 1e300;
     for (int v = 0; v < 3; v++) {
       if (ms) ms[3 * c + v] = -1.0;
-      if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0) || (c >= 5 && h->colpass)) continue;
+      // the sweeps are timed where they are in use, the one-pass shapes where that is
+      if (nv_ < 2 || v >= nv_ || (c < 5 && nw <= 0) || (c >= 5 && c < 8 && h->colpass) || (c == 8 && !h->colpass))
+        continue;
       auto launch = [&]() -> int {
         if (c < 5) return dnsk::launch_plane_kernel(a, nw, h->fp32, c, v, h->stream);
+        if (c == 8) return dnsk::launch_column_pass(ca, v, h->stream);
         if (c == 5) dnsk::launch_fused_up(ca, v, h->stream);
         else if (c == 6) dnsk::launch_fused_down(ca, v, h->stream);
         else dnsk::launch_fused_final(ca, v, h->stream);
@@ -1118,7 +1123,7 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
       if (ms) ms[3 * c + v] = tv;
       if (tv < tbest) { tbest = tv; best = v; }
     }
-    if (c >= 5 && h->colpass) best = h->var[c];  // sweeps not in use: keep the current choice
+    if ((c >= 5 && c < 8 && h->colpass) || (c == 8 && !h->colpass)) best = h->var[c];  // path not in use
     h->var[c] = best;
     if (chosen) chosen[c] = best;
   }

@@ -211,7 +211,8 @@ int four_to_phys_scratch(void **ptr);
 // one-pass column stage (column_pass.hip): 8 <= nyu <= 512 (with flag_ctpress = 0 followed by
 // launch_meanflow_post)
 bool column_pass_supported(int nyu);
-int launch_column_pass(const ColArgs &a, hipStream_t st);  // 0, or 1 when nyu is not covered
+int column_pass_variants(int nyu);  // workgroup shapes of nyu's bucket (same arithmetic, same bits)
+int launch_column_pass(const ColArgs &a, int variant, hipStream_t st);  // 0, or 1 when nyu is not covered
 // ---- ensemble of several sizes: grouped launches (ensemble.hip, column_pass.hip) ----
 constexpr int ENS_MAXG = 4;          // size groups per ensemble
 constexpr int ENS_PLANE_NTH = 128;   // block size of the grouped plane launches

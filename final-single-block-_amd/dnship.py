@@ -270,8 +270,8 @@ class DNS:
                     substep=t[6], n=n.value)
 
     # ---- plane-kernel tile variants and the autotuner
-    KERNELS = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v", "col_up", "col_down", "col_final")
-    NKERN = 8
+    KERNELS = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v", "col_up", "col_down", "col_final", "col_pass")
+    NKERN = 9
 
     def plane_variants(self):
         """(current variant per kernel, number of variants per kernel), int arrays of NKERN."""

@@ -291,19 +291,19 @@ module dns_hip
     integer(c_int) function dns_get_plane_variants(h, var, count) bind(C, name='dns_get_plane_variants')
       import :: c_int, c_ptr
       type(c_ptr), value :: h
-      integer(c_int) :: var(8), count(8)
+      integer(c_int) :: var(9), count(9)
     end function
     integer(c_int) function dns_set_plane_variants(h, var) bind(C, name='dns_set_plane_variants')
       import :: c_int, c_ptr
       type(c_ptr), value :: h
-      integer(c_int), intent(in) :: var(8)
+      integer(c_int), intent(in) :: var(9)
     end function
     integer(c_int) function dns_autotune(h, iters, chosen, ms) bind(C, name='dns_autotune')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       integer(c_int), value :: iters
-      integer(c_int) :: chosen(8)
-      real(c_double) :: ms(24)
+      integer(c_int) :: chosen(9)
+      real(c_double) :: ms(27)
     end function
     function c_strlen(s) bind(C, name='strlen')
       import :: c_ptr, c_size_t

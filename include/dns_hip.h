@@ -224,13 +224,14 @@ int dns_get_timing(dns_handle *h, double *t, int *n);
 
 /* Kernel variants, index c: 0 z-inverse, 1 x-pass of u-planes, 2 x-pass of v-planes, 3 z-forward of
  * u-planes, 4 z-forward of v-planes (tile shapes), 5 column sweep up (thread per column or one
- * wavefront per solve), 6 column sweep down, 7 final sweep.  All variants of a kernel give
+ * wavefront per solve), 6 column sweep down, 7 final sweep, 8 one-pass column stage (columns per
+ * workgroup at the bucket's segment count and length).  All variants of a kernel give
  * bit-identical results.  No counterpart in the reference (one code path per subroutine); they stand
  * in for the loop orders of four_to_phys_du / phys_to_four_du (FOU3D.f90:857-1231) and of the solves
  * (tridLU_3D.f90:38-221).
  * dns_get_plane_variants: var[DNS_NKERN] current choice, count[DNS_NKERN] (may be NULL) variants available.
  * dns_set_plane_variants: var[DNS_NKERN]; status 1 for a variant that does not exist. */
-#define DNS_NKERN 8
+#define DNS_NKERN 9
 int dns_get_plane_variants(dns_handle *h, int *var, int *count);
 int dns_set_plane_variants(dns_handle *h, const int *var);
 /* Autotuner: times each variant of each kernel that has several (iters launches; plane kernels on the

@@ -273,10 +273,10 @@ def test_plane_tile_variants_bit_identical(prec, Mx, Mz, ny):
     """Every tile variant of every plane kernel gives the same bits (tuned lengths have three)."""
     st = dns_ic.seeded_channel(Mx, Mz, ny, seed=4, broadband=True)
     ref = None
-    for var in ([0] * 8, [1] * 8, [2] * 8, [1, 2, 0, 2, 1, 2, 1, 2]):
+    for var in ([0] * 9, [1] * 9, [2] * 9, [1, 2, 0, 2, 1, 2, 1, 2, 0]):
         g = dnship.DNS(Mx, Mz, ny, precision=prec, plane_batch=11, column_path=1)
         cur, cnt = g.plane_variants()
-        assert list(cur) == [0] * 8 and list(cnt) == [3] * 8
+        assert list(cur) == [0] * 9 and list(cnt) == [3] * 9
         print("case: variants %s prec %d" % (var, prec), flush=True)
         g.set_plane_variants(var)
         g.upload(*st)
@@ -299,7 +299,7 @@ def test_autotune_choice_valid_and_results_unchanged():
     g.upload(*st)
     ch, ms = g.autotune(iters=2)
     assert np.all((ch >= 0) & (ch < 3))
-    assert np.all(ms > 0)
+    assert np.all(ms[:8] > 0) and np.all(ms[8] == -1)    # the one-pass shapes are not in use here
     assert list(g.plane_variants()[0]) == list(ch)
     assert all(np.array_equal(x, y) for x, y in zip(g.download(), st))   # fields restored
     sc = g.scalars()
@@ -313,10 +313,44 @@ def test_autotune_choice_valid_and_results_unchanged():
         assert np.array_equal(x, y)
     h = dnship.DNS(48, 48, 33)
     _, cnt = h.plane_variants()
-    assert list(cnt) == [1] * 5 + [3, 3, 3]
+    assert list(cnt) == [1] * 5 + [3, 3, 3, 3]
     with pytest.raises(dnship.DnsError):
-        h.set_plane_variants([0, 1, 0, 0, 0, 0, 0, 0])
+        h.set_plane_variants([0, 1, 0, 0, 0, 0, 0, 0, 0])
     h.close()
+
+
+@pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
+@pytest.mark.parametrize("Mx,Mz,ny", [(48, 48, 33), (48, 48, 153), (24, 24, 257), (24, 24, 385)])
+def test_column_pass_shapes_bit_identical_and_tuned(prec, Mx, Mz, ny):
+    """The one-pass column stage's workgroup shapes of one row bucket (columns per workgroup; same segment
+    count and length) give the same bits, and the autotuner times them on the one-pass path."""
+    st = dns_ic.seeded_channel(Mx, Mz, ny, seed=17, broadband=True)
+    ref = None
+    g = dnship.DNS(Mx, Mz, ny, precision=prec)
+    nv = g.plane_variants()[1][8]
+    g.close()
+    assert nv >= 2
+    for v in range(nv):
+        g = dnship.DNS(Mx, Mz, ny, precision=prec)
+        g.set_plane_variants([0] * 8 + [v])
+        g.upload(*st)
+        g.step(2)
+        out = g.download() + g.download_nonlinear()
+        g.close()
+        if ref is None:
+            ref = out
+        else:
+            for x, y in zip(ref, out):
+                assert np.array_equal(x, y), v
+    g = dnship.DNS(Mx, Mz, ny, precision=prec)
+    g.upload(*st)
+    ch, ms = g.autotune(iters=2)
+    assert np.all(ms[8, :nv] > 0) and np.all(ms[5:8] == -1) and 0 <= ch[8] < nv
+    g.upload(*st)
+    g.step(2)
+    for x, y in zip(ref, g.download() + g.download_nonlinear()):
+        assert np.array_equal(x, y)
+    g.close()
 
 
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
