@@ -1,7 +1,8 @@
 ! Drop-in Fortran driver: the reference's main loop (littleharsh.f90:163-233) calling subroutines with the
 ! reference's names and argument lists, each of which forwards to the HIP implementation through the
-! ISO_C_BINDING interface (dns_hip_mod.f90).  Fields live in GPU memory between calls; the host arrays in
-! the argument lists keep the reference's shapes and are synchronised only for output.
+! ISO_C_BINDING interface (dns_hip_mod.f90; wrappers in lh_wrappers.f90).  Fields live in GPU memory between
+! calls; the host arrays in the argument lists keep the reference's shapes and are synchronised for output
+! (and after every wrapper call with DNS_HIP_MIRROR=1).
 !
 !   littleharsh_hip [nsteps] [fused]
 !
@@ -14,95 +15,6 @@
 ! (start.f90:410, FOU3D.f90:77-81) the plane statistics and spectra are accumulated on the device; at each
 ! write step they go to ``<dirout>/stats_NNNNxNNNNxNNNN_TTTTT.dat`` (stats.f90:244-413) and
 ! ``<dirout>/sp{U,V,W,UV,P}_NNNNxNNNNxNNNN_TTTTT.dat`` (spectra.f90:71-188) and are reset.
-
-module lh_state
-  use iso_c_binding
-  use dns_hip
-  implicit none
-  type(c_ptr) :: hdns = c_null_ptr
-  integer :: kRK = 1, solveU_calls = 0
-  integer :: Nspec_x, Nspec_z, Ngal_x, Ngal_z, nyu, nyv, ny, ncol, flag_ctpress, flag_init, nwrite, ppp
-  real(8) :: Re, alp, bet, CFL, mpgx, dyq, maxt, t, dt
-  integer :: iter, iwrite, iter0, nstat
-  character(len=120) :: fnameimb, dirin, dirout
-end module
-
-! ---- wrappers with the reference's names (littleharsh.f90:682-993) ----
-subroutine RHS0_u1(du1, u1, Nu1, p, myid)
-  use lh_state
-  implicit none
-  integer myid
-  complex(8) du1(*), u1(*), Nu1(*), p(*)
-  solveU_calls = 0
-  call dns_hip_check(dns_rhs0(hdns, int(kRK, c_int)), 'RHS0_u1')   ! RHS0 of u1, u2 and u3
-end subroutine
-
-subroutine RHS0_u2(du2, u2, Nu2, p, myid)
-  implicit none
-  integer myid
-  complex(8) du2(*), u2(*), Nu2(*), p(*)
-  ! computed together with RHS0_u1 on the GPU
-end subroutine
-
-subroutine RHS0_u3(du3, u3, Nu3, p, myid)
-  implicit none
-  integer myid
-  complex(8) du3(*), u3(*), Nu3(*), p(*)
-  ! computed together with RHS0_u1 on the GPU
-end subroutine
-
-subroutine nonlinear(Nu1, Nu2, Nu3, u1, u2, u3, du1, du2, du3, p, div, myid, status, ierr)
-  use lh_state
-  implicit none
-  integer myid, status(*), ierr
-  complex(8) Nu1(*), Nu2(*), Nu3(*), u1(*), u2(*), u3(*), du1(*), du2(*), du3(*), p(*), div(*)
-  call dns_hip_check(dns_nonlinear(hdns, int(kRK, c_int)), 'nonlinear')
-  ierr = 0
-end subroutine
-
-subroutine solveU(u, du, grid, a_grid, myid)
-  use lh_state
-  implicit none
-  integer grid, myid
-  complex(8) u(*), du(*)
-  real(8) a_grid(*)
-  ! the three velocity solves of a substep run in one launch at the first call
-  if (solveU_calls == 0) call dns_hip_check(dns_solveU(hdns, int(kRK, c_int)), 'solveU')
-  solveU_calls = solveU_calls + 1
-end subroutine
-
-subroutine meanflow_ctU(u1, myid)
-  use lh_state
-  implicit none
-  integer myid
-  complex(8) u1(*)
-  call dns_hip_check(dns_meanflow_ctU(hdns, int(kRK, c_int)), 'meanflow_ctU')
-end subroutine
-
-subroutine solveP(p, psi, u1, u2, u3, div, myid)
-  use lh_state
-  implicit none
-  integer myid
-  complex(8) p(*), psi(*), u1(*), u2(*), u3(*), div(*)
-  call dns_hip_check(dns_solveP(hdns, int(kRK, c_int)), 'solveP')
-end subroutine
-
-subroutine v_corr(u1, u2, u3, psi, div, myid, status, ierr)
-  use lh_state
-  implicit none
-  integer myid, status(*), ierr
-  complex(8) u1(*), u2(*), u3(*), psi(*), div(*)
-  call dns_hip_check(dns_vcorr(hdns, int(kRK, c_int)), 'v_corr')
-  ierr = 0
-end subroutine
-
-subroutine meanflow_ctP(u1, myid)
-  use lh_state
-  implicit none
-  integer myid
-  complex(8) u1(*)
-  call dns_hip_check(dns_meanflow_ctP(hdns), 'meanflow_ctP')
-end subroutine
 
 ! ---------------------------------------------------------------------------------------------------
 program littleharsh_hip
@@ -129,6 +41,7 @@ program littleharsh_hip
     fused = trim(arg) == 'fused'
   end if
 
+  call lh_mirror_init()
   call read_input('input.in')
   prm%Mx = Ngal_x; prm%Mz = Ngal_z; prm%ny = ny; prm%ppp = ppp
   prm%flag_ctpress = flag_ctpress; prm%precision = DNS_PREC_FP64

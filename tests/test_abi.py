@@ -118,3 +118,15 @@ def test_read_input_in():
     cfg = dns_io.read_input(os.path.join(ROOT, "final-single-block-_amd", "fortran", "input.in"))
     assert cfg["Mx"] == 48 and cfg["ny"] == 153 and cfg["flag_ctpress"] == 1
     assert cfg["mpgx"] == -3.0675e-3 and cfg["CFL"] == 0.7
+
+
+def test_fortran_wrappers_stop_out_of_order_calls():
+    """The drop-in wrappers keep the fields on the GPU and advance them only in the reference's call order
+    (littleharsh.f90:165-222); a substep that starts with RHS0_u2 stops with a message naming the call
+    expected, before anything reaches the GPU."""
+    import subprocess
+    prog = os.path.join(ROOT, "final-single-block-_amd", "fortran", "lh_order_check")
+    assert os.path.exists(prog), "build the Fortran programs first (__graft_entry__.build())"
+    r = subprocess.run([prog], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert "RHS0_u2 called out of the reference's order; expected RHS0_u1" in r.stderr
