@@ -70,7 +70,7 @@ def gpu_leg(st, prec, steps, warmup, size=None, Re=RE, mpgx=MPGX):
 ENSEMBLE = [(24, 48, 48, 153), (16, 96, 96, 97), (8, 192, 192, 153)]
 
 
-def ensemble_leg(steps, warmup, prec):
+def ensemble_leg(steps, warmup, prec, eager=False):
     """The 48-member ensemble (three sizes) with one grouped launch per kernel class per RK3 substep for
     all members, the step replayed from a captured hipGraph (value); beside it the same members as three
     handles on three streams, each handle's step autotuned and replayed from its own graph."""
@@ -101,10 +101,11 @@ def ensemble_leg(steps, warmup, prec):
                 g.upload(*st, member=m)
 
     ens = dnship.Ensemble(handles)
-    ens.step_graph(max(warmup, 1))
+    ens_step = ens.step if eager else ens.step_graph
+    ens_step(max(warmup, 1))
     ens.sync()
     t0 = time.perf_counter()
-    ens.step_graph(steps)
+    ens_step(steps)
     ens.sync()
     sec = (time.perf_counter() - t0) / steps
     reupload()
@@ -118,12 +119,12 @@ def ensemble_leg(steps, warmup, prec):
         g.autotune(iters=3)
     reupload()
     for g in handles:
-        g.step_graph(max(warmup, 1))
+        (g.step if eager else g.step_graph)(max(warmup, 1))
     for g in handles:
         g.sync()
     t0 = time.perf_counter()
     for g in handles:
-        g.step_graph(steps)
+        (g.step if eager else g.step_graph)(steps)
     for g in handles:
         g.sync()
     sec_streams = (time.perf_counter() - t0) / steps
@@ -131,8 +132,8 @@ def ensemble_leg(steps, warmup, prec):
         g.close()
     return {"value": 1.0 / sec, "unit": "ensemble steps/s (all 48 members advance one RK3 step)",
             "ms_per_step": sec * 1e3, "members": idx, "groups": [list(x) for x in ENSEMBLE],
-            "launch": "one grouped launch per kernel class per substep for all members, step replayed from a "
-                      "hipGraph",
+            "launch": "one grouped launch per kernel class per substep for all members, "
+                      + ("eager steps" if eager else "step replayed from a hipGraph"),
             "kernels_ms_per_substep": {k: v for k, v in tim.items() if k != "n"},
             "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
             "three_streams": {"value": 1.0 / sec_streams, "ms_per_step": sec_streams * 1e3,
@@ -261,7 +262,7 @@ def error_leg(st, nsteps):
 SMALL = [(48, 48, 153), (192, 192, 153)]
 
 
-def small_leg(steps, warmup):
+def small_leg(steps, warmup, eager=False):
     """FP64 and mixed steps/s of single channels at the smaller configurations, each RK3 step replayed from
     a captured hipGraph (at these sizes the launches, not HBM, set the pace without one)."""
     import dnship
@@ -277,10 +278,11 @@ def small_leg(steps, warmup):
             g.upload(*st)
             g.autotune(iters=3)
             g.upload(*st)
-            g.step_graph(max(warmup, 1))
+            step = g.step if eager else g.step_graph
+            step(max(warmup, 1))
             g.sync()
             t0 = time.perf_counter()
-            g.step_graph(steps)
+            step(steps)
             g.sync()
             sec = (time.perf_counter() - t0) / steps
             g.close()
@@ -351,6 +353,9 @@ def main():
     ap.add_argument("--no-post", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="step the ensemble and small legs eagerly instead of replaying captured hipGraphs "
+                         "(for kernel traces)")
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="the mixed path's error against FP64 after this many steps, with the drift of a "
                          "1e-10-perturbed FP64 run beside it (0: leave out)")
@@ -424,13 +429,13 @@ def main():
         if args.error_steps > 0:
             out["mixed"]["long_run_error"] = error_leg(st, args.error_steps)
     if not args.no_ensemble:
-        out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64)}
+        out["ensemble"] = {"fp64": ensemble_leg(args.steps, args.warmup, dnship.PREC_FP64, args.eager)}
         if not args.no_mixed:
-            out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED)
+            out["ensemble"]["mixed"] = ensemble_leg(args.steps, args.warmup, dnship.PREC_MIXED, args.eager)
     if not args.no_large:
         out["large"] = large_leg(max(1, min(args.steps, 5)), 1)
     if not args.no_small:
-        out["small"] = small_leg(max(args.steps, 20) * 5, args.warmup)
+        out["small"] = small_leg(max(args.steps, 20) * 5, args.warmup, args.eager)
     if not args.no_post:
         out["four_to_phys"] = post_leg(f64[0])
         out["stats"] = stats_leg(st)
