@@ -47,6 +47,7 @@ struct dns_handle {
   int var[DNS_NKERN] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool colpass = false;  // column stage: one-pass kernel (true) or the three sweeps
   double *stat_acc = nullptr, *stat_sp = nullptr, *stat_part = nullptr;  // statistics (allocated on first use)
+  double2 *p_stat = nullptr;       // the pressure Pm, P2m sum after dns_refresh_stats_pressure (ppPL)
   int istat = 0;
   size_t bytes = 0;
   hipStream_t stream = nullptr;
@@ -1168,11 +1169,24 @@ extern "C" int dns_accumulate_stats(dns_handle *h) {
   a.nyu = h->nyu; a.nyv = h->nyv; a.nm = h->nm; a.nchunk = nchunk;
   a.bet = h->prm.bet; a.gi1 = h->gi1; a.gi2 = h->gi2;
   a.yu = h->d_yu; a.yv = h->d_yv; a.dthdyv = h->d_dthdyv;
-  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p;
+  a.u1 = h->u1; a.u2 = h->u2; a.u3 = h->u3; a.p = h->p; a.pm = h->p_stat ? h->p_stat : h->p;
   a.partial = h->stat_part; a.acc = h->stat_acc; a.sp = h->stat_sp;
   dnsk::launch_stats(a, h->stream);
   HIPCHK(hipGetLastError());
   h->istat++;
+  return DNS_OK;
+}
+
+// ppPL = 0; modes_to_planes_UVP(ppPL, p) (FOU3D.f90:149-152): the plane sums Pm, P2m read this copy of
+// every member's p from now on; the reference refreshes it at write steps and at the end of the run
+// (littleharsh.f90:439-443), and reads it from the initial field file (start.f90:1369-1387).
+extern "C" int dns_refresh_stats_pressure(dns_handle *h) {
+  if (!h) return fail(DNS_ERR_ARG, "dns_refresh_stats_pressure: null handle");
+  for (int m = 0; m < h->nm; m++)
+    if (!h->uploaded[m]) return fail(DNS_ERR_STATE, "dns_refresh_stats_pressure: no fields uploaded for member %d", m);
+  const size_t n = (size_t)h->nm * dnsk::member_strides(h->C, h->nyu, h->nyv).rp;
+  if (!h->p_stat && dalloc(h, &h->p_stat, n)) return fail(DNS_ERR_HIP, "dns_refresh_stats_pressure: out of device memory");
+  HIPCHK(hipMemcpyAsync(h->p_stat, h->p, n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
   return DNS_OK;
 }
 

@@ -184,6 +184,7 @@ struct StatsArgs {
   double bet, gi1, gi2;
   const double *yu, *yv, *dthdyv;
   const double2 *u1, *u2, *u3, *p;
+  const double2 *pm;  // pressure summed into Pm, P2m: p, or the copy ppPL holds (dns_refresh_stats_pressure)
   double *partial;  // [m][j][chunk][q]
   double *acc;      // [m][q][j], j = 0..nyu+1
   double *sp;       // spectra [m]: spU (nyu+2 rows), spV (nyv+2), spW (nyu+2), spUV (nyu+2), spP (nyu), rows x C

@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
   const int chunk = blockIdx.x, j = blockIdx.y, m = blockIdx.z;
   const MemberStrides ms = member_strides(a.C, a.nyu, a.nyv);
   StatsArgs b = a;
-  b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp;
+  b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp; b.pm += m * ms.rp;
   const int nyu = a.nyu;
   const bool uin = j >= 1 && j <= nyu;          // u and v planes in physical space
   const size_t C = a.C, nu = (size_t)(nyu + 2) * C, nv = (size_t)(a.nyv + 2) * C;
@@ -154,12 +154,13 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
         s[10] += 0.5 * (re_dot(w, w) + re_mul(w, wp));
       }
     }
-    // ---- p planes 1..nyu: spectral layout
+    // ---- p planes 1..nyu: spectral layout; the spectrum of p, the plane sums of the ppPL pressure
     if (uin) {
       const double2 x = b.p[(size_t)(j - 1) * a.C + col];
       spP[(size_t)(j - 1) * C + col] = spP[(size_t)(j - 1) * C + col] + x.x * x.x + x.y * x.y;
-      s[11] += sum_ri(x);
-      s[12] += re_dot(x, x);
+      const double2 xm = b.pm[(size_t)(j - 1) * a.C + col];
+      s[11] += sum_ri(xm);
+      s[12] += re_dot(xm, xm);
     }
   }
   // block reduction (fixed order), one partial per (member, plane, chunk)

@@ -46,7 +46,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_autotune", "dns_upload_member", "dns_download_member", "dns_set_member_params",
            "dns_get_member_scalars", "dns_set_column_path", "dns_get_column_path",
            "dns_history_record", "dns_accumulate_stats", "dns_get_stats", "dns_reset_stats",
-           "dns_four_to_phys", "dns_four_to_phys_noalloc", "dns_ensemble_create", "dns_ensemble_destroy",
+           "dns_refresh_stats_pressure", "dns_four_to_phys", "dns_four_to_phys_noalloc", "dns_ensemble_create",
+           "dns_ensemble_destroy",
            "dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_sync", "dns_ensemble_set_timing",
            "dns_ensemble_get_timing"]
 
@@ -95,6 +96,7 @@ def lib():
     L.dns_history_record.argtypes = [vp, ctypes.c_int, _dp]
     L.dns_accumulate_stats.argtypes = [vp]
     L.dns_reset_stats.argtypes = [vp]
+    L.dns_refresh_stats_pressure.argtypes = [vp]
     L.dns_get_stats.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.dns_four_to_phys.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp]
     L.dns_four_to_phys_noalloc.argtypes = [ctypes.c_int] * 5 + [vp, vp, vp]
@@ -313,6 +315,11 @@ class DNS:
 
     def reset_stats(self):
         _chk(lib().dns_reset_stats(self.h))
+
+    def refresh_stats_pressure(self):
+        """ppPL = modes_to_planes_UVP(p) (FOU3D.f90:149-152): Pm, P2m sum this copy of the current p from
+        now on, until the next call (the reference refreshes it at write steps)."""
+        _chk(lib().dns_refresh_stats_pressure(self.h))
 
     def stats(self, member=0, spectra=False):
         """dict: istat, the 13 plane statistics (arrays of nyu+2; Pm, P2m at index j = 1..nyu+1) and, with

@@ -177,6 +177,11 @@ module dns_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    ! ppPL = modes_to_planes_UVP(p) (FOU3D.f90:149-152): Pm, P2m sum this copy until the next call
+    integer(c_int) function dns_refresh_stats_pressure(h) bind(C, name='dns_refresh_stats_pressure')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
     ! stats(nyu+2, 13) in stats.f90's order; spectra: spU, spV, spW, spUV, spP one after another, each
     ! sp(rows, columns) as in spectra.f90 (rows nyu+2, nyv+2, nyu+2, nyu+2, nyu); c_null_ptr skips either
     integer(c_int) function dns_get_stats(h, m, stats, spectra, istat) bind(C, name='dns_get_stats')

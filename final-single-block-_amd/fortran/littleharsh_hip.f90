@@ -62,6 +62,7 @@ program littleharsh_hip
     iter = 0
   end if
   call dns_hip_check(dns_upload_fields(hdns, u1, u2, u3, p, t, int(iter, c_int), 0d0), 'getini')
+  call dns_hip_check(dns_refresh_stats_pressure(hdns), 'getini')   ! ppPL from the field file (start.f90:1387)
   iter_start = iter
   iwrite = iter
   iter0 = iter
@@ -104,6 +105,7 @@ program littleharsh_hip
   write(*,'(A,I0,A,ES24.16,A,ES24.16,A,ES24.16)') ' iter ', iter, '  t = ', sc(1), '  dt = ', sc(2), &
     '  Qx = ', sc(5)
   iwrite = iter
+  call dns_hip_check(dns_refresh_stats_pressure(hdns), 'finalize')   ! littleharsh.f90:439-443
   call record_out_hip(u1, u2, u3, p)
   close(30)
   call dns_hip_check(dns_destroy(hdns), 'finalize')
@@ -297,14 +299,18 @@ contains
   end subroutine
 
   ! the kRK = 1 output of FOU3D.f90:77-90, 138-154, 338-348: statistics every nstat steps; at write steps
-  ! record_out, then the statistics and spectra files, after which the sums start again
+  ! ppPL takes the current p before the statistics (FOU3D.f90:149-152 precede stats at :340), then
+  ! record_out and the statistics and spectra files, after which the sums start again
   subroutine step_output(u1, u2, u3, p)
     complex(8), intent(inout) :: u1(0:nyu+1, ncol), u2(0:nyv+1, ncol), u3(0:nyu+1, ncol), p(1:nyu, ncol)
+    logical :: flagwr
+    flagwr = iter >= iwrite
+    if (flagwr) call dns_hip_check(dns_refresh_stats_pressure(hdns), 'ppPL')
     if (iter - iter0 >= nstat) then
       iter0 = iter
       call dns_hip_check(dns_accumulate_stats(hdns), 'stats')
     end if
-    if (iter >= iwrite) then
+    if (flagwr) then
       call record_out_hip(u1, u2, u3, p)
       call write_stats_files()
       call dns_hip_check(dns_reset_stats(hdns), 'write_stats')

@@ -154,12 +154,20 @@ int dns_history_record(dns_handle *h, int m, double *rec);
  * planes 0..nyu+1), Vm, V2m, VWm, wxm, wx2m (v planes 0..nyu+1), Pm, P2m (p planes 1..nyu+1 at index j,
  * index 0 unused), each array nyu+2 long; spectra (may be NULL) = spU [ncol][nyu+2], spV [ncol][nyv+2],
  * spW, spUV [ncol][nyu+2], spP [ncol][nyu] in the reference's sp(j, column) layout.  Pm, P2m sum the
- * spectral p planes that record_out leaves in ppPL, i.e. the values of a statistics step that is also a
- * write step.  The physical-plane sums are formed with the exact discrete sum rules of the reference's
- * transforms (agreement with the reference to rounding, tested on its own output). */
+ * spectral p planes of ppPL (stats.f90:44-51): the current p until dns_refresh_stats_pressure is first
+ * called, the copy it took from then on.  The physical-plane sums are formed with the exact discrete sum
+ * rules of the reference's transforms (agreement with the reference to rounding, tested on its own
+ * output). */
 int dns_accumulate_stats(dns_handle *h);
 int dns_get_stats(dns_handle *h, int m, double *stats, double *spectra, int *istat);
 int dns_reset_stats(dns_handle *h);
+/* ppPL = 0; modes_to_planes_UVP(ppPL, p) (FOU3D.f90:149-152; littleharsh.f90:439-443): copies every
+ * member's current p into the pressure that Pm, P2m sum until the next call.  The reference fills ppPL from
+ * the initial field file (start.f90:1369-1387) and refreshes it at each write step before that step's
+ * statistics, so a statistics step that is not a write step sums the pressure of the last write step;
+ * a driver reproduces that with one call after the upload and one at each write step before
+ * dns_accumulate_stats. */
+int dns_refresh_stats_pressure(dns_handle *h);
 
 /* four_to_phys post-processing: the per-plane resize + cft + rft of the reference's post-processors
  * (four_to_phys.f90:60-81 and four_to_phys_N.f90:60-81, resize at four_to_phys.f90:318-337), for a batch of

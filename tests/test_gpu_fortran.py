@@ -126,6 +126,55 @@ def test_fortran_driver_stats_files_vs_oracle(tmp_path):
     assert seen, files
 
 
+def test_fortran_driver_stats_pressure_of_last_write(tmp_path):
+    """nwrite = 40, so nstat = min(20, nwrite) = 20: statistics at steps 20 and 40, files at 0 and 40.  Pm
+    and P2m sum ppPL, which the reference refreshes only at write steps, before that step's statistics
+    (FOU3D.f90:149-152 precede stats at :340): step 20 sums the pressure of step 0, step 40 its own.
+    Every other statistic sums the state of its own step."""
+    Mx, Mz, ny, n = 48, 48, 33, 41
+    st, tag = write_case(str(tmp_path), Mx, Mz, ny, seed=14)
+    path = os.path.join(str(tmp_path), "input.in")
+    with open(path) as f:
+        lines = f.read().split("\n")
+    lines[34] = "nwrt =%11d" % 40
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+    print("case: fortran driver statistics nwrite=40 %dx%dx%d %d steps" % (Mx, Mz, ny, n), flush=True)
+    r = subprocess.run([DRIVER, str(n), "fused"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    outdir = os.path.join(str(tmp_path), "output")
+    files = [dns_io.read_stats(os.path.join(outdir, f)) for f in os.listdir(outdir) if f.startswith("stats_")]
+    assert sorted(s["iter"] for s in files) == [0, 40]
+    s = [s for s in files if s["iter"] == 40][0]
+    assert s["istat"] == 2
+    o = pyoracle.Oracle(Mx, Mz, ny)
+    grid = o.setup()
+    o.set_state(*st)
+
+    def stats_now():
+        return pyoracle.plane_stats(Mx, Mz, ny, *o.get_state(), grid["yu"], grid["yv"], grid["dthdyv"], 2.0)
+
+    p0 = stats_now()
+    o.step(20)
+    s20 = stats_now()
+    o.step(20)
+    s40 = stats_now()
+    nyu = ny - 1
+    exp = {}
+    for k in dns_io.STATS_FILE_ORDER:
+        if k in ("Pm", "P2m"):
+            exp[k] = (p0[k] + s40[k])[1:nyu + 1]
+        elif k in ("Vm", "V2m", "VWm", "wxm", "wx2m"):
+            exp[k] = (s20[k] + s40[k])[:nyu + 1]
+        else:
+            exp[k] = s20[k] + s40[k]
+    err = stats_errors(s, exp, Mx * Mz, (2 * Mx // 3) * (2 * Mz // 3))
+    assert max(err.values()) <= 1e-11, err
+    # the pressure of step 20 itself differs: the test sees the difference it is about
+    cur = (s20["Pm"] + s40["Pm"])[1:nyu + 1]
+    assert np.abs(cur - exp["Pm"]).max() > 1e3 * 1e-11 * np.sqrt(np.abs(exp["P2m"]).max())
+
+
 def test_fortran_driver_rejects_bad_size(tmp_path):
     write_case(str(tmp_path), 48, 48, 33, seed=1)
     with open(os.path.join(str(tmp_path), "input.in")) as f:
