@@ -3,6 +3,7 @@
 // one unit per kernel class and precision, so they compile in parallel).
 #pragma once
 #include "plane_kernels.h"
+#include "xpass_reg.h"
 
 namespace dnsk {
 
@@ -12,7 +13,8 @@ namespace dnsk {
 // picks among them.  Plans and tiles of the tuned lengths were chosen by timing on MI355X:
 //   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column),
 //     16 columns per workgroup (256-byte row segments in FP64);
-//   x-pass half-length 288: 16.18 (two passes).
+//   x-pass at 576 and 1152 points: the register x-pass (xpass_reg.h), 9 points per lane and 32 or 64
+//     lanes per row; other lengths: the LDS x-pass with the plans of XPlanT.
 #define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
 #define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
 
@@ -34,10 +36,10 @@ template <int N, int V> struct ZTile {
   static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (NB < 64 ? 64 : NB > 256 ? 256 : NB);
   static constexpr int ZCH = (N == 576 && V == 1) ? 4 : (N == 576 && V == 3) ? 8 : 2;
 };
-// x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront per
-// row (variant 2's barriers are wave-local).  1152 points (one row's slots take 39 KB of LDS, so only four
-// rows fit a CU): one row per workgroup with 2, 4 or 1 wavefronts.  Shorter rows have too few butterflies
-// per pass for a wavefront, so RPW rows share one: workgroups of 2, 1 or 4 wavefronts.
+// LDS x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront
+// per row.  1152 points: one row per workgroup with 2, 4 or 1 wavefronts.  (Both lengths run the register
+// x-pass, xreg_v; the ensemble's grouped launch keeps these tiles for its lengths.)  Shorter rows have too
+// few butterflies per pass for a wavefront, so RPW rows share one: workgroups of 2, 1 or 4 wavefronts.
 template <int MX, int V> struct XTile {
   static constexpr int H = MX / 2;
   static constexpr int RPW = H <= 48 ? 4 : H <= 96 ? 2 : 1;   // rows per wavefront below 576 points
@@ -69,11 +71,26 @@ inline void zfwd_v(const PlaneArgs &a, int nw, hipStream_t st) {
     zfwd_kernel<R, MZ, Z::T, Z::NTH, VPL, typename ZPlanT<MZ>::type, 1, 0><<<g, Z::NTH, 0, st>>>(a);
   }
 }
+// register x-pass (xpass_reg.h): variants 0, 1, 2 = 4, 2, 1 wavefronts per workgroup
+template <typename R, int MX, int V, bool VPL>
+inline void xreg_v(const PlaneArgs &a, int nw, hipStream_t st) {
+  constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 64;
+  using B = XregBody<R, MX, NTH, VPL>;
+  if (a.Mz % B::RB != 0) {
+    using B1 = XregBody<R, MX, 64, VPL>;  // Mz is even; one wavefront holds 64 / L rows
+    xreg_kernel<R, MX, 64, VPL><<<dim3(a.Mz / B1::RB, nw * a.nm), 64, 0, st>>>(a);
+    return;
+  }
+  xreg_kernel<R, MX, NTH, VPL><<<dim3(a.Mz / B::RB, nw * a.nm), NTH, 0, st>>>(a);
+}
+
 template <typename R, int MX, int V, bool VPL>
 inline void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
   using X = XTile<MX, V>;
   constexpr int H = MX / 2;
-  if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
+  if constexpr (xreg_enabled(MX)) {
+    xreg_v<R, MX, V, VPL>(a, nw, st);
+  } else if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
     xpass_v<R, MX, 0, VPL>(a, nw, st);
   } else {
     using PL = typename XPlanT<H>::type;

@@ -1,0 +1,286 @@
+// Register-resident x-pass (the x-direction half of four_to_phys_u / phys_to_four_du and the products
+// of ops_in_planes2, FOU3D.f90:1514-1652, rft_buff.f:62-84) for rows whose packed length H = MX/2 is
+// RR * L with RR = 1, 3 or 9 and L = 2..64 a power of two.
+//
+// One row transform per group of L lanes, RR points per lane, no LDS in the transforms:
+//   inverse (sign +1), input Z[L n1 + t] in lane t, register n1 (the row's memory order, so the loads
+//     are coalesced): RR-point DFT in registers, twiddle e^{+2 pi i t m1/H}, then an L-point DFT across
+//     the lanes as radix-2 decimation-in-frequency stages (partner lane t ^ h, values exchanged with DPP
+//     for h = 1, 2 and ds_swizzle / ds_bpermute above).  Lane t ends with z[m1 + RR brev(t)]: bit-reversed blocks of RR physical points.
+//   forward (sign -1), input in that same layout: radix-2 decimation-in-time stages across the lanes
+//     (bit-reversed input, natural output k2 = t), twiddle e^{-2 pi i m1 t/H}, RR-point DFT in registers.
+//     Lane t ends with Z[L k1 + t], the inverse's input layout, so the outputs are stored coalesced.
+// The products are pointwise and every field of a row has the same layout, so a lane forms them from
+// its own registers.  The only cross-lane traffic besides the butterflies is the Hermitian partner
+// Z[H - k] of the real-to-complex post-processing, in lane (L - t) mod L (ds_bpermute).
+// Pre- and post-processing are the packed real transform of the LDS x-pass (same formulas, same order
+// of operations); the transforms themselves round differently from the LDS Stockham engine.
+#pragma once
+#include "plane_kernels.h"
+
+namespace dnsk {
+
+template <int H> struct XregShape {
+  static constexpr int L = H & -H;  // largest power of two dividing H
+  static constexpr int RR = H / L;
+  static constexpr bool ok = L >= 2 && L <= 64 && (RR == 1 || RR == 3 || RR == 9);
+};
+// x lengths that use the register x-pass (all of its tile variants, so every variant gives the same bits)
+__host__ __device__ constexpr bool xreg_enabled(int MX) { return MX == 576 || MX == 1152; }
+
+// value of lane (lane ^ M) of the wavefront, M < 64 a power of two
+template <int M> __device__ __forceinline__ int lane_xor_i(int v) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+  } else if constexpr (M == 4 || M == 8 || M == 16) {
+    // ds_swizzle bit mode within 32 lanes (and 0x1f, or 0, xor M): the exchange runs on the LDS unit,
+    // which the transforms otherwise leave idle, instead of taking VALU issue slots
+    return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));
+  } else {
+    static_assert(M == 32, "lane exchange");
+    return __builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 63) ^ 32) << 2), v);
+  }
+}
+template <int M> __device__ __forceinline__ double lane_xor(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor_i<M>((int)(unsigned)(b & 0xffffffffLL)), hi = lane_xor_i<M>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int M> __device__ __forceinline__ float lane_xor(float v) {
+  return __int_as_float(lane_xor_i<M>(__float_as_int(v)));
+}
+template <int M, typename T> __device__ __forceinline__ T lane_xor_c(T v) {
+  T c;
+  c.x = lane_xor<M>(v.x);
+  c.y = lane_xor<M>(v.y);
+  return c;
+}
+// value of lane src (byte address src * 4) of the wavefront
+__device__ __forceinline__ double lane_read(double v, int addr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float lane_read(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+template <typename R, int MX, int NTH_, bool VPL>
+struct XregBody {
+  static constexpr int NTH = NTH_;
+  static constexpr int H = MX / 2;
+  using S = XregShape<H>;
+  static constexpr int L = S::L, RR = S::RR;
+  static_assert(S::ok, "register x-pass: H = RR L, RR in {1, 3, 9}, L in 2..64");
+  static constexpr int GPW = 64 / L;            // rows per wavefront
+  static constexpr int RB = (NTH / 64) * GPW;   // rows per workgroup
+  static constexpr int LOGL = L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : L == 32 ? 5 : 6;
+  static constexpr int NSTG = LOGL - 1;         // butterfly stages with twiddles (half-size h >= 2)
+  static constexpr int NSTASH = VPL ? 0 : RR;   // u-planes keep u's physical values in LDS meanwhile
+  static constexpr size_t LDS = sizeof(cx<R>) * NSTASH * NTH + sizeof(double) * (NTH / 64);
+  static constexpr int Kx = MX / 3 + 1;
+  static constexpr int KOUT = (Kx + L - 1) / L;  // registers holding outputs k < Kx
+  __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
+};
+
+template <typename R, int MX, int NTH_, bool VPL>
+__device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs &a_, int lb, int gx, int gy,
+                                                               unsigned char *lds) {
+  (void)gy;
+  const int bx = lb % gx, by = lb / gx;
+  const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
+  const int jj = by % a.nbj, j = a.j0 + jj;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int t = lane & (L - 1), g = lane / L;
+  const int z = bx * RB + wv * GPW + g;  // this lane group's row
+  const int MZ = a.Mz;
+  constexpr int Kp = row_pitch<R>(Kx);
+  const size_t fsz = (size_t)MZ * Kp;
+  const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);
+  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + pl * 3 * fsz + (size_t)z * Kp;
+  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + pl * 4 * fsz + (size_t)z * Kp;
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}
+  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}, then twr
+  const cx<R> *twr = twh + H;                                  // twr[(m1-1) L + t] = e^{-2 pi i t m1/H}
+  cx<R> *stash = reinterpret_cast<cx<R> *>(lds);
+  double *red = reinterpret_cast<double *>(lds + sizeof(cx<R>) * NSTASH * NTH);
+
+  // butterfly twiddles of this lane: stage s (half-size h = 2 << s), e^{-2 pi i (t mod h)/(2h)} on the
+  // upper lane of each pair, 1 on the lower
+  cx<R> ws[NSTG > 0 ? NSTG : 1];
+#pragma unroll
+  for (int s = 0; s < NSTG; s++) {
+    const int h = 2 << s;
+    ws[s] = (t & h) ? twh[(t & (h - 1)) * (H / (2 * h))] : mk<R>(1, 0);
+  }
+
+  // packed half-length spectrum of field f: v[n1] = Z[L n1 + t]
+  auto load_inverse = [&](int f, cx<R> *v) {
+    const cx<R> *row = I + f * fsz;
+#pragma unroll
+    for (int n1 = 0; n1 < RR; n1++) {
+      const int n = L * n1 + t;
+      cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
+      if (n < Kx) xa = row[n];
+      if (n == 0) xa.y = 0;
+      if (n > 0 && H - n < Kx) xb = row[H - n];
+      const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
+      const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
+      v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
+    }
+  };
+  // inverse transform: in-register DFT, twiddles, cross-lane DIF; lane t ends with z[m1 + RR brev(t)]
+  auto inverse = [&](cx<R> *v) {
+    if constexpr (RR > 1) {
+      dft<R, RR, +1>::run(v);
+#pragma unroll
+      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmulc(v[m1], twr[(m1 - 1) * L + t]);
+    }
+    sfor<0, LOGL>([&](auto si) {
+      constexpr int h = L >> (decltype(si)::value + 1);
+      const R sg = (t & h) ? (R)-1 : (R)1;
+#pragma unroll
+      for (int i = 0; i < RR; i++) {
+        const cx<R> p = lane_xor_c<h>(v[i]);
+        cx<R> d = mk<R>(fma(sg, v[i].x, p.x), fma(sg, v[i].y, p.y));  // lower: v + p, upper: p - v
+        if constexpr (h >= 2) {
+          constexpr int s = (h == 2 ? 0 : h == 4 ? 1 : h == 8 ? 2 : h == 16 ? 3 : 4);
+          d = cmulc(d, ws[s]);
+        }
+        v[i] = d;
+      }
+    });
+  };
+  // forward transform: cross-lane DIT, twiddles, in-register DFT; lane t ends with Z[L k1 + t]
+  auto forward = [&](cx<R> *v) {
+    sfor<0, LOGL>([&](auto si) {
+      constexpr int h = 1 << decltype(si)::value;
+      const R sg = (t & h) ? (R)-1 : (R)1;
+#pragma unroll
+      for (int i = 0; i < RR; i++) {
+        cx<R> m = v[i];
+        if constexpr (h >= 2) {
+          constexpr int s = (h == 2 ? 0 : h == 4 ? 1 : h == 8 ? 2 : h == 16 ? 3 : 4);
+          m = cmul(m, ws[s]);
+        }
+        const cx<R> p = lane_xor_c<h>(m);
+        v[i] = mk<R>(fma(sg, m.x, p.x), fma(sg, m.y, p.y));  // lower: m + p, upper: p - m
+      }
+    });
+    if constexpr (RR > 1) {
+#pragma unroll
+      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmul(v[m1], twr[(m1 - 1) * L + t]);
+      dft<R, RR, -1>::run(v);
+    }
+  };
+  // real-to-complex post-processing and store of the retained modes k < Kx of product slot p
+  const R sc = (R)(0.5 / (double)MX);
+  const int paddr = ((lane & ~(L - 1)) | ((L - t) & (L - 1))) << 2;  // lane (L - t) mod L of the group
+  auto store = [&](int p, const cx<R> *v) {
+    cx<R> *out = P + (size_t)p * MZ * Kp;
+#pragma unroll
+    for (int k1 = 0; k1 < KOUT; k1++) {
+      cx<R> q;
+      q.x = lane_read(v[RR - 1 - k1].x, paddr);
+      q.y = lane_read(v[RR - 1 - k1].y, paddr);
+      if (t == 0) q = v[(RR - k1) % RR];  // Z[H - L k1] is this lane's own
+      const int k = L * k1 + t;
+      if (k < Kx) {
+        const cx<R> zk = v[k1], zm = cconj(q);
+        const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
+        cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
+        x = cscale(x, sc);
+        if (k == 0) x.y = 0;
+        out[k] = x;
+      }
+    }
+  };
+  auto sq = [&](cx<R> *v) {
+#pragma unroll
+    for (int i = 0; i < RR; i++) v[i] = mk<R>(v[i].x * v[i].x, v[i].y * v[i].y);
+  };
+  auto times = [&](cx<R> *v, const cx<R> *w) {
+#pragma unroll
+    for (int i = 0; i < RR; i++) v[i] = mk<R>(v[i].x * w[i].x, v[i].y * w[i].y);
+  };
+
+  cx<R> A[RR], B[RR];
+  if (!VPL) {
+    const bool cfl = (a.kRK == 1) && (j <= a.nyu - 1);
+    R m1 = 0, m3 = 0;
+    load_inverse(1, A);  // v: vv
+    inverse(A);
+    sq(A);
+    forward(A);
+    store(2, A);
+    load_inverse(0, A);  // u, kept in LDS (each lane its own slots)
+    inverse(A);
+#pragma unroll
+    for (int i = 0; i < RR; i++) {
+      if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
+      stash[i * NTH + threadIdx.x] = A[i];
+    }
+    load_inverse(2, B);  // w
+    inverse(B);
+#pragma unroll
+    for (int i = 0; i < RR; i++) {
+      if (cfl) m3 = fmax(m3, fmax(fabs(B[i].x), fabs(B[i].y)));
+      A[i] = mk<R>(A[i].x * B[i].x, A[i].y * B[i].y);  // uw
+    }
+    forward(A);
+    store(1, A);
+    sq(B);  // ww
+    forward(B);
+    store(3, B);
+#pragma unroll
+    for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + threadIdx.x];
+    sq(A);  // uu
+    forward(A);
+    store(0, A);
+    if (a.kRK == 1 && j <= a.nyu - 1) {  // uniform over the workgroup
+      const double b1 = block_max<NTH>((double)m1, red);
+      const double b3 = block_max<NTH>((double)m3, red);
+      if (threadIdx.x == 0) {  // spread over CFL_SLOTS addresses: one address for every block serialises
+        const int slot = (bx + by) & (CFL_SLOTS - 1);
+        atomic_max_nonneg(&a.cfl[slot], b1);
+        atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);
+      }
+    }
+  } else {
+    load_inverse(1, A);  // v
+    inverse(A);
+    load_inverse(0, B);  // u: uv
+    inverse(B);
+    times(B, A);
+    forward(B);
+    store(0, B);
+    load_inverse(2, B);  // w: wv
+    inverse(B);
+    times(B, A);
+    forward(B);
+    store(1, B);
+  }
+}
+
+template <typename R, int MX, int NTH, bool VPL, int WPE = 1>
+__global__ void __launch_bounds__(NTH, WPE) xreg_kernel(PlaneArgs a) {
+  using B = XregBody<R, MX, NTH, VPL>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  B::run(a, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, lds);
+}
+
+// twr[(m1 - 1) L + t] = e^{-2 pi i t m1/H} (m1 = 1..RR-1, t < L) after the table e^{-2 pi i m/H}: copies of
+// table entries, so both give the same bits
+template <typename R, int H>
+__global__ void xreg_twiddles_kernel(cx<R> *tw) {
+  using S = XregShape<H>;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < H; m += gridDim.x * blockDim.x) {
+    const int m1 = m / S::L + 1, t = m % S::L;
+    tw[H + m] = m1 < S::RR ? tw[(t * m1) % H] : mk<R>(0, 0);
+  }
+}
+
+}  // namespace dnsk
