@@ -11,8 +11,9 @@
 //     (bit-reversed input, natural output k2 = t), twiddle e^{-2 pi i m1 t/H}, RR-point DFT in registers.
 //     Lane t ends with Z[L k1 + t], the inverse's input layout, so the outputs are stored coalesced.
 // The products are pointwise and every field of a row has the same layout, so a lane forms them from
-// its own registers.  The only cross-lane traffic besides the butterflies is the Hermitian partner
-// Z[H - k] of the real-to-complex post-processing, in lane (L - t) mod L (ds_bpermute).
+// its own registers.  The only cross-lane traffic besides the butterflies is the Hermitian partners
+// (X[H - n] of the inverse's pre-processing, Z[H - k] of the forward's post-processing), in lane
+// (L - t) mod L (ds_bpermute).
 // Pre- and post-processing are the packed real transform of the LDS x-pass (same formulas, same order
 // of operations); the transforms themselves round differently from the LDS Stockham engine.
 #pragma once
@@ -117,16 +118,34 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
     ws[s] = (t & h) ? twh[(t & (h - 1)) * (H / (2 * h))] : mk<R>(1, 0);
   }
 
-  // packed half-length spectrum of field f: v[n1] = Z[L n1 + t]
+  // lane (L - t) mod L of this lane's group, as a ds_bpermute address: it holds the Hermitian partners
+  const int paddr = ((lane & ~(L - 1)) | ((L - t) & (L - 1))) << 2;
+  // packed half-length spectrum of field f: v[n1] = Z[L n1 + t].  In FP32 the row's modes are loaded once
+  // (X[L n1 + t], zero from Kx on) and the partner X[H - n] = X[L (RR-1-n1) + (L-t)] is taken from
+  // register RR-1-n1 of lane L - t (for t = 0: X[L (RR - n1)], this lane's register RR - n1); in FP64,
+  // where the exchange costs two ds_bpermute per component, it is loaded again.  Same values either way.
   auto load_inverse = [&](int f, cx<R> *v) {
     const cx<R> *row = I + f * fsz;
+    constexpr bool XCH = sizeof(R) == 4;
+    cx<R> xr[RR];
 #pragma unroll
     for (int n1 = 0; n1 < RR; n1++) {
       const int n = L * n1 + t;
-      cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
-      if (n < Kx) xa = row[n];
+      xr[n1] = n < Kx ? row[n] : mk<R>(0, 0);
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < RR; n1++) {
+      const int n = L * n1 + t;
+      cx<R> xb = mk<R>(0, 0);
+      if constexpr (XCH) {
+        xb.x = lane_read(xr[RR - 1 - n1].x, paddr);
+        xb.y = lane_read(xr[RR - 1 - n1].y, paddr);
+        if (t == 0) xb = n1 == 0 ? mk<R>(0, 0) : xr[(RR - n1) % RR];
+      } else {
+        if (n > 0 && H - n < Kx) xb = row[H - n];
+      }
+      cx<R> xa = xr[n1];
       if (n == 0) xa.y = 0;
-      if (n > 0 && H - n < Kx) xb = row[H - n];
       const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
       const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
       v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
@@ -178,7 +197,6 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   };
   // real-to-complex post-processing and store of the retained modes k < Kx of product slot p
   const R sc = (R)(0.5 / (double)MX);
-  const int paddr = ((lane & ~(L - 1)) | ((L - t) & (L - 1))) << 2;  // lane (L - t) mod L of the group
   auto store = [&](int p, const cx<R> *v) {
     cx<R> *out = P + (size_t)p * MZ * Kp;
 #pragma unroll
