@@ -6,8 +6,8 @@
 // column_pass.hip.  Every member runs the reference's single-channel RK3 loop (littleharsh.f90:163-233).
 //
 // Tiles: 128-thread workgroups for the plane kernels at every length (z-kernels: 32, 16, 8 columns at
-// 48, 96, 192 points; x-pass: 8, 8, 4 rows), so that one launch has one block size.  All tile variants of
-// a kernel compute the same bits, so a member's result equals the same channel run alone.
+// 48, 96, 192 points; the register x-pass: 16, 8, 4 rows), so that one launch has one block size.  All
+// tile variants of a kernel compute the same bits, so a member's result equals the same channel run alone.
 #include "plane_launch.h"
 
 namespace dnsk {
@@ -34,8 +34,11 @@ using EXpass = XpassBody<R, N, EnsX<N>::X::RB, ENS_PLANE_NTH, VPL, typename XPla
 // which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; body of length N
 template <typename R, int WHICH, int N> struct EnsBody;
 template <typename R, int N> struct EnsBody<R, 0, N> { using B = EZinv<R, N>; };
-template <typename R, int N> struct EnsBody<R, 1, N> { using B = EXpass<R, N, false>; };
-template <typename R, int N> struct EnsBody<R, 2, N> { using B = EXpass<R, N, true>; };
+// x-pass: the single-channel launch's engine for the length (register x-pass where enabled), 128 threads
+template <typename R, int N, bool VPL>
+using EX = typename std::conditional<xreg_enabled(N), XregBody<R, N, ENS_PLANE_NTH, VPL>, EXpass<R, N, VPL>>::type;
+template <typename R, int N> struct EnsBody<R, 1, N> { using B = EX<R, N, false>; };
+template <typename R, int N> struct EnsBody<R, 2, N> { using B = EX<R, N, true>; };
 template <typename R, int N> struct EnsBody<R, 3, N> { using B = EZfwd<R, N, false>; };
 template <typename R, int N> struct EnsBody<R, 4, N> { using B = EZfwd<R, N, true>; };
 
@@ -74,16 +77,16 @@ static void ens_plane_grid(const PlaneArgs &a, int which, int &gx, int &gy, int 
   const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
   int T = 0, ZCH = 1, RB = 1;
   switch (n) {
-    case 48: T = EnsZ<48>::Z::T; ZCH = EnsZ<48>::Z::ZCH; RB = EnsX<48>::X::RB; break;
-    case 96: T = EnsZ<96>::Z::T; ZCH = EnsZ<96>::Z::ZCH; RB = EnsX<96>::X::RB; break;
-    default: T = EnsZ<192>::Z::T; ZCH = EnsZ<192>::Z::ZCH; RB = EnsX<192>::X::RB; break;
+    case 48: T = EnsZ<48>::Z::T; ZCH = EnsZ<48>::Z::ZCH; RB = EX<double, 48, false>::ROWS; break;
+    case 96: T = EnsZ<96>::Z::T; ZCH = EnsZ<96>::Z::ZCH; RB = EX<double, 96, false>::ROWS; break;
+    default: T = EnsZ<192>::Z::T; ZCH = EnsZ<192>::Z::ZCH; RB = EX<double, 192, false>::ROWS; break;
   }
   if (which == 0) {
     gx = (a.Kx + T - 1) / T;
     gy = 3;
     nblk = gx * gy * ((a.nbj + ZCH - 1) / ZCH) * a.nm;
   } else if (which <= 2) {
-    gx = a.Mz / RB;
+    gx = (a.Mz + RB - 1) / RB;
     gy = a.nbj * a.nm;
     nblk = gx * gy;
   } else {
@@ -94,8 +97,9 @@ static void ens_plane_grid(const PlaneArgs &a, int which, int &gx, int &gy, int 
 }
 
 int ens_plane_rows_ok(int Mx, int Mz) {
-  const int rb = Mx == 48 ? EnsX<48>::X::RB : Mx == 96 ? EnsX<96>::X::RB : EnsX<192>::X::RB;
-  return Mz % rb == 0;
+  const int rb = Mx == 48 ? EX<double, 48, false>::ROWS : Mx == 96 ? EX<double, 96, false>::ROWS
+                                                      : EX<double, 192, false>::ROWS;
+  return (xreg_enabled(Mx) || Mz % rb == 0) ? 1 : 0;  // the register x-pass takes any Mz
 }
 
 int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, hipStream_t st) {

@@ -137,6 +137,7 @@ struct XpassBody {
   static constexpr int H = MX / 2;
   static constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
   static constexpr int NS = RB * NSL;
+  static constexpr int ROWS = RB;  // rows per body
   // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers; twiddles from
   // the pass-blocked copy after twh (contiguous loads across the butterflies of a pass)
   using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false, true>;  // inverse: active f*RB + r -> slot r*NSL + f

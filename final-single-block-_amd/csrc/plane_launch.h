@@ -76,12 +76,7 @@ template <typename R, int MX, int V, bool VPL>
 inline void xreg_v(const PlaneArgs &a, int nw, hipStream_t st) {
   constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 64;
   using B = XregBody<R, MX, NTH, VPL>;
-  if (a.Mz % B::RB != 0) {
-    using B1 = XregBody<R, MX, 64, VPL>;  // Mz is even; one wavefront holds 64 / L rows
-    xreg_kernel<R, MX, 64, VPL><<<dim3(a.Mz / B1::RB, nw * a.nm), 64, 0, st>>>(a);
-    return;
-  }
-  xreg_kernel<R, MX, NTH, VPL><<<dim3(a.Mz / B::RB, nw * a.nm), NTH, 0, st>>>(a);
+  xreg_kernel<R, MX, NTH, VPL><<<dim3((a.Mz + B::RB - 1) / B::RB, nw * a.nm), NTH, 0, st>>>(a);
 }
 
 template <typename R, int MX, int V, bool VPL>

@@ -27,7 +27,9 @@ template <int H> struct XregShape {
   static constexpr bool ok = L >= 2 && L <= 64 && (RR == 1 || RR == 3 || RR == 9);
 };
 // x lengths that use the register x-pass (all of its tile variants, so every variant gives the same bits)
-__host__ __device__ constexpr bool xreg_enabled(int MX) { return MX == 576 || MX == 1152; }
+__host__ __device__ constexpr bool xreg_enabled(int MX) {
+  return MX == 48 || MX == 96 || MX == 192 || MX == 576 || MX == 1152;
+}
 
 // value of lane (lane ^ M) of the wavefront, M < 64 a power of two
 template <int M> __device__ __forceinline__ int lane_xor_i(int v) {
@@ -78,6 +80,7 @@ struct XregBody {
   static_assert(S::ok, "register x-pass: H = RR L, RR in {1, 3, 9}, L in 2..64");
   static constexpr int GPW = 64 / L;            // rows per wavefront
   static constexpr int RB = (NTH / 64) * GPW;   // rows per workgroup
+  static constexpr int ROWS = RB;
   static constexpr int LOGL = L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : L == 32 ? 5 : 6;
   static constexpr int NSTG = LOGL - 1;         // butterfly stages with twiddles (half-size h >= 2)
   static constexpr int NSTASH = VPL ? 0 : RR;   // u-planes keep u's physical values in LDS meanwhile
@@ -96,8 +99,12 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   const int jj = by % a.nbj, j = a.j0 + jj;
   const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
   const int t = lane & (L - 1), g = lane / L;
-  const int z = bx * RB + wv * GPW + g;  // this lane group's row
   const int MZ = a.Mz;
+  // this lane group's row; groups past the last row (Mz not a multiple of RB) redo the last row and store
+  // nothing (its lanes still take part in the wavefront's exchanges)
+  const int zr = bx * RB + wv * GPW + g;
+  const bool zok = zr < MZ;
+  const int z = zok ? zr : MZ - 1;
   constexpr int Kp = row_pitch<R>(Kx);
   const size_t fsz = (size_t)MZ * Kp;
   const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);
@@ -206,7 +213,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
       q.y = lane_read(v[RR - 1 - k1].y, paddr);
       if (t == 0) q = v[(RR - k1) % RR];  // Z[H - L k1] is this lane's own
       const int k = L * k1 + t;
-      if (k < Kx) {
+      if (k < Kx && zok) {
         const cx<R> zk = v[k1], zm = cconj(q);
         const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
         cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
