@@ -133,29 +133,38 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   // where the exchange costs two ds_bpermute per component, it is loaded again.  Same values either way.
   auto load_inverse = [&](int f, cx<R> *v) {
     const cx<R> *row = I + f * fsz;
-    constexpr bool XCH = sizeof(R) == 4;
-    cx<R> xr[RR];
+    if constexpr (sizeof(R) == 4) {
+      cx<R> xr[RR];
 #pragma unroll
-    for (int n1 = 0; n1 < RR; n1++) {
-      const int n = L * n1 + t;
-      xr[n1] = n < Kx ? row[n] : mk<R>(0, 0);
-    }
+      for (int n1 = 0; n1 < RR; n1++) {
+        const int n = L * n1 + t;
+        xr[n1] = n < Kx ? row[n] : mk<R>(0, 0);
+      }
 #pragma unroll
-    for (int n1 = 0; n1 < RR; n1++) {
-      const int n = L * n1 + t;
-      cx<R> xb = mk<R>(0, 0);
-      if constexpr (XCH) {
+      for (int n1 = 0; n1 < RR; n1++) {
+        const int n = L * n1 + t;
+        cx<R> xb;
         xb.x = lane_read(xr[RR - 1 - n1].x, paddr);
         xb.y = lane_read(xr[RR - 1 - n1].y, paddr);
         if (t == 0) xb = n1 == 0 ? mk<R>(0, 0) : xr[(RR - n1) % RR];
-      } else {
-        if (n > 0 && H - n < Kx) xb = row[H - n];
+        cx<R> xa = xr[n1];
+        if (n == 0) xa.y = 0;
+        const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
+        const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
+        v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
       }
-      cx<R> xa = xr[n1];
-      if (n == 0) xa.y = 0;
-      const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
-      const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
-      v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < RR; n1++) {
+        const int n = L * n1 + t;
+        cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
+        if (n < Kx) xa = row[n];
+        if (n == 0) xa.y = 0;
+        if (n > 0 && H - n < Kx) xb = row[H - n];
+        const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
+        const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
+        v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
+      }
     }
   };
   // inverse transform: in-register DFT, twiddles, cross-lane DIF; lane t ends with z[m1 + RR brev(t)]
