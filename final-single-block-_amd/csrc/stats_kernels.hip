@@ -77,12 +77,15 @@ __device__ __forceinline__ long partner(const StatsArgs &a, size_t col, bool pad
   return (long)kk * a.Kx + i;
 }
 
+// SNAP: Pm, P2m read a separate copy of the pressure (dns_refresh_stats_pressure); otherwise they sum p itself
+template <bool SNAP>
 __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
   __shared__ double red[STAT_NTH / 64][STAT_NQ];
   const int chunk = blockIdx.x, j = blockIdx.y, m = blockIdx.z;
   const MemberStrides ms = member_strides(a.C, a.nyu, a.nyv);
   StatsArgs b = a;
-  b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp; b.pm += m * ms.rp;
+  b.u1 += m * ms.ru; b.u3 += m * ms.ru; b.u2 += m * ms.rv; b.p += m * ms.rp;
+  if (SNAP) b.pm += m * ms.rp;
   const int nyu = a.nyu;
   const bool uin = j >= 1 && j <= nyu;          // u and v planes in physical space
   const size_t C = a.C, nu = (size_t)(nyu + 2) * C, nv = (size_t)(a.nyv + 2) * C;
@@ -158,7 +161,8 @@ __global__ void __launch_bounds__(STAT_NTH) plane_stats_kernel(StatsArgs a) {
     if (uin) {
       const double2 x = b.p[(size_t)(j - 1) * a.C + col];
       spP[(size_t)(j - 1) * C + col] = spP[(size_t)(j - 1) * C + col] + x.x * x.x + x.y * x.y;
-      const double2 xm = b.pm[(size_t)(j - 1) * a.C + col];
+      double2 xm = x;  // without a separate copy, the ppPL pressure is p itself (loaded once)
+      if constexpr (SNAP) xm = b.pm[(size_t)(j - 1) * a.C + col];
       s[11] += sum_ri(xm);
       s[12] += re_dot(xm, xm);
     }
@@ -197,7 +201,8 @@ __global__ void stats_finish_kernel(StatsArgs a) {
 int stats_nchunk(int C) { return (C + STAT_CHUNK - 1) / STAT_CHUNK; }
 
 void launch_stats(const StatsArgs &a, hipStream_t st) {
-  plane_stats_kernel<<<dim3(a.nchunk, a.nyu + 2, a.nm), STAT_NTH, 0, st>>>(a);
+  if (a.pm != a.p) plane_stats_kernel<true><<<dim3(a.nchunk, a.nyu + 2, a.nm), STAT_NTH, 0, st>>>(a);
+  else plane_stats_kernel<false><<<dim3(a.nchunk, a.nyu + 2, a.nm), STAT_NTH, 0, st>>>(a);
   const int n = a.nm * (a.nyu + 2) * STAT_NQ;
   stats_finish_kernel<<<(n + 255) / 256, 256, 0, st>>>(a);
 }
