@@ -139,25 +139,42 @@ TRAJ_COLS = np.array([0, 1, 2, 64, 65, 130, 1000, 4159, 4160, 8319])
 TRAJ_STOPS = (10, 100, 1000)
 
 
+TRAJ_CHUNK = 100  # steps per reference run (each run must finish inside run_reference's time limit)
+
+
 def traj128_run(pert: bool):
     """The reference's own code from the 128x128x153 seeded IC (seed 1, amplitude 0.1, lowest 8 modes;
-    pert: every velocity value times 1 + 1e-15 N(0,1), rng seed 1), CFL steps, chained runs stopping at
-    TRAJ_STOPS: per-field norms and the TRAJ_COLS columns at each stop, and the per-step (t, dt) history."""
+    pert: every velocity value times 1 + 1e-15 N(0,1), rng seed 1), CFL steps, as chained runs of at most
+    TRAJ_CHUNK steps (each run starts from the fields the last one wrote, which carries the whole state at
+    a constant pressure gradient): per-field norms and the TRAJ_COLS columns at each of TRAJ_STOPS, and the
+    per-step (t, dt) history.  After every run the state is checkpointed in the temporary directory, so an
+    interrupted generation resumes where it stopped."""
     Mx, Mz, ny = 192, 192, 153
+    ckpt = os.path.join(tempfile.gettempdir(), "traj128_%s.npz" % ("pert" if pert else "plain"))
     st = list(dns_ic.seeded_channel(Mx, Mz, ny, seed=1))
     if pert:
         rng = np.random.default_rng(1)
         st = [a * (1 + 1e-15 * rng.standard_normal(a.shape)) for a in st[:3]] + [st[3]]
-    out = {}
-    t, it, done, hist = 0.0, 0, 0, []
-    for stop in TRAJ_STOPS:
-        r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=stop - done, t=t, it=it)
+    out, t, done, hist = {}, 0.0, 0, []
+    if os.path.exists(ckpt):
+        c = np.load(ckpt, allow_pickle=False)
+        st = [c[k] for k in ("u1", "u2", "u3", "p")]
+        t, done = float(c["t"]), int(c["done"])
+        hist = [c["hist"]] if c["hist"].size else []
+        out = {k[4:]: c[k] for k in c.files if k.startswith("out_")}
+        print("traj128 %s: resumed at %d steps" % ("perturbed" if pert else "plain", done), flush=True)
+    while done < TRAJ_STOPS[-1]:
+        stop = min([x for x in TRAJ_STOPS if x > done] + [done + TRAJ_CHUNK])
+        r = pyoracle.run_reference(Mx, Mz, ny, *st, mode=1, nsteps=stop - done, t=t, it=done)
         st = [r[k] for k in ("u1", "u2", "u3", "p")]
-        t, it, done = r["t"], it + stop - done, stop
+        t, done = r["t"], stop
         hist.append(r["hist"])
-        for k, a in zip(("u1", "u2", "u3", "p"), st):
-            out["norm_%s_%d" % (k, stop)] = np.linalg.norm(a)
-            out["cols_%s_%d" % (k, stop)] = a[TRAJ_COLS]
+        if stop in TRAJ_STOPS:
+            for k, a in zip(("u1", "u2", "u3", "p"), st):
+                out["norm_%s_%d" % (k, stop)] = np.linalg.norm(a)
+                out["cols_%s_%d" % (k, stop)] = a[TRAJ_COLS]
+        np.savez(ckpt, u1=st[0], u2=st[1], u3=st[2], p=st[3], t=t, done=done, hist=np.concatenate(hist),
+                 **{"out_" + k: v for k, v in out.items()})
         print("traj128 %s: %d steps" % ("perturbed" if pert else "plain", stop), flush=True)
     out["hist"] = np.concatenate(hist)
     return out
