@@ -252,14 +252,11 @@ __host__ __device__ constexpr int next_radix(int rem) {
 // active transforms, e.g. all rows of field 0 first).
 // OPQ = false lets the optimiser keep the passes' index arithmetic in registers (for kernels whose
 // occupancy LDS limits, so that registers are not the scarce resource).
-// PTW = true reads the butterflies' twiddles from the pass-blocked copy that follows the table
-// (pass_twiddles_kernel): with consecutive butterflies on consecutive threads, a pass's twiddle loads are
-// then contiguous instead of strided by r entries.
 template <int N_, int NTR_, int NACT_ = NTR_, int GA_ = 1, int GS_ = 1, int PS_ = 0, bool BF_ = false,
-          bool TR_ = false, bool OPQ_ = true, bool PTW_ = false>
+          bool TR_ = false, bool OPQ_ = true>
 struct Lay {
   static constexpr int N = N_, NTR = NTR_, NACT = NACT_, PS = PS_;
-  static constexpr bool BF = BF_, PTW = PTW_;
+  static constexpr bool BF = BF_;
   static constexpr int LR = PS > 0 ? N + (N >> PS) : N;  // rows per slot
   static constexpr int SIZE = NTR * LR;                   // elements
   __device__ __forceinline__ static int row(int n) { return PS > 0 ? n + (n >> PS) : n; }
@@ -292,20 +289,19 @@ __device__ __forceinline__ int tidx() {
   return t;
 }
 
-template <int N_, int NTR_, int NACT_, int GA_, int GS_, int PS_, bool BF_, bool TR_, bool OPQ_, bool PTW_>
-__device__ __forceinline__ int Lay<N_, NTR_, NACT_, GA_, GS_, PS_, BF_, TR_, OPQ_, PTW_>::tid() {
+template <int N_, int NTR_, int NACT_, int GA_, int GS_, int PS_, bool BF_, bool TR_, bool OPQ_>
+__device__ __forceinline__ int Lay<N_, NTR_, NACT_, GA_, GS_, PS_, BF_, TR_, OPQ_>::tid() {
   return OPQ_ ? tidx() : (int)threadIdx.x;
 }
 
-// Butterfly of one pass on registers u[0..RAD): twiddle (P > 1), DFT.  PTW: the twiddle of output r of
-// butterfly k is the pass-blocked copy tw[N + r*P + k] of tw[r*k*N/(P*RAD)] (same value).
-template <typename R, int N, int S, int P, int RAD, bool PTW = false>
+// Butterfly of one pass on registers u[0..RAD): twiddle (P > 1), DFT.
+template <typename R, int N, int S, int P, int RAD>
 __device__ __forceinline__ void butterfly(cx<R> *u, int k, const cx<R> *__restrict__ tw) {
   constexpr int TSTEP = N / (P * RAD);
   if (P > 1) {
 #pragma unroll
     for (int r = 1; r < RAD; r++) {
-      const cx<R> w = PTW ? tw[N + r * P + k] : tw[r * k * TSTEP];
+      const cx<R> w = tw[r * k * TSTEP];
       u[r] = (S < 0) ? cmul(u[r], w) : cmulc(u[r], w);
     }
   }
@@ -339,7 +335,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
       L::template split<M>(b, a, i);
       const int t = L::slot(a);
       const int k = i % P;
-      butterfly<R, L::N, S, P, RAD, L::PTW>(u[q], k, tw);
+      butterfly<R, L::N, S, P, RAD>(u[q], k, tw);
       const int j0 = (i - k) * RAD + k;
 #pragma unroll
       for (int r = 0; r < RAD; r++) s[L::addr(j0 + r * P, t)] = u[q][r];
@@ -455,7 +451,7 @@ __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict
       cx<R> u[RAD];
 #pragma unroll
       for (int r = 0; r < RAD; r++) u[r] = s[L::addr(i + r * P, t)];
-      butterfly<R, L::N, S, P, RAD, L::PTW>(u, i, tw);
+      butterfly<R, L::N, S, P, RAD>(u, i, tw);
 #pragma unroll
       for (int r = 0; r < RAD; r++) st(i + r * P, a, q * RAD + r, u[r]);
     }
@@ -486,36 +482,6 @@ __device__ __forceinline__ void fft_from_mem(cx<R> *s, const cx<R> *__restrict__
   static_assert(I::prod == L::N && I::len >= 2, "plan does not match the length / fewer than two passes");
   fft_first<R, L, NTH, S, I::first>(s, ld);
   fft_passes<R, L, NTH, S, I::first, typename I::tail>::run(s, tw);
-}
-
-// Pass (span P, radix RAD) of plan Rs... whose blocked twiddles hold entry m: the pass with
-// P <= m < P*RAD (P = 0 for m below the first radix: the first pass has no twiddles).
-template <int... Rs>
-__host__ __device__ inline void twiddle_pass_of(Plan<Rs...>, int m, int &P, int &RAD) {
-  const int rad[] = {Rs...};
-  int p = 1;
-  P = 0;
-  RAD = 0;
-  for (int i = 0; i < (int)sizeof...(Rs); i++) {
-    if (p > 1 && m >= p && m < p * rad[i]) {
-      P = p;
-      RAD = rad[i];
-    }
-    p *= rad[i];
-  }
-}
-
-// Pass-blocked copy of the twiddle table tw[0..N) of plan PL, written to tw[N..2N): the pass of span
-// P and radix RAD keeps its twiddles at tw[N + r*P + k] = tw[r*k*N/(P*RAD)] (output r = 1..RAD-1 of
-// butterfly k = 0..P-1); these blocks of successive passes tile [N + first radix, 2N).  Values are
-// copied, so transforms that read them give the same bits.
-template <typename R, int N, class PL>
-__global__ void pass_twiddles_kernel(cx<R> *tw) {
-  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < N; m += gridDim.x * blockDim.x) {
-    int P, RAD;
-    twiddle_pass_of(PL{}, m, P, RAD);
-    tw[N + m] = P > 0 ? tw[(m / P) * (m % P) * (N / (P * RAD))] : mk<R>(0, 0);
-  }
 }
 
 // ------------------------------------------------------------------------------------------------

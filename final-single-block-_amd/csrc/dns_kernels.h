@@ -150,8 +150,8 @@ int xpass_rows_per_block(int Mx);  // rows an x-pass launch needs Mz to be a mul
 int plane_variants(int n);          // tile variants instantiated for transform length n
 // which: 0 z-inverse (nw = nbj + 1 field rows), 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes (nw = nbj)
 int launch_plane_kernel(const PlaneArgs &a, int nw, bool fp32, int which, int variant, hipStream_t st);
-// fills the second half of the x-pass twiddle table twh (2 * Mx/2 entries, plane precision): the LDS
-// x-pass's pass-blocked copy, or the register x-pass's lane twiddles
+// fills the second half of the x-pass twiddle table twh (2 * Mx/2 entries, plane precision): the
+// register x-pass's lane twiddles
 int build_xpass_twiddles(void *twh, int Mx, bool fp32, hipStream_t st);
 void launch_cfl_reduce(const double *slots, DnsState *out, int nm, hipStream_t st);
 void launch_cfl_reset(DnsState *st, double *slots, int nm, hipStream_t s);

@@ -18,25 +18,15 @@ template <int N> struct EnsZ {   // z-kernel tile: ZTile variant with 128 thread
   using Z = ZTile<N, V>;
   static_assert(Z::NTH == ENS_PLANE_NTH, "one block size per launch");
 };
-template <int N> struct EnsX {   // x-pass tile: XTile variant 0 (128 threads at these lengths)
-  using X = XTile<N, 0>;
-  static_assert(X::NTH == ENS_PLANE_NTH, "one block size per launch");
-};
-
 template <typename R, int N> using EZinv = ZinvBody<R, N, EnsZ<N>::Z::T, ENS_PLANE_NTH, typename ZPlanT<N>::type,
                                                     EnsZ<N>::Z::ZCH>;
 template <typename R, int N, bool VPL>
 using EZfwd = ZfwdBody<R, N, EnsZ<N>::Z::T, ENS_PLANE_NTH, VPL, typename ZPlanT<N>::type>;
-template <typename R, int N, bool VPL>
-using EXpass = XpassBody<R, N, EnsX<N>::X::RB, ENS_PLANE_NTH, VPL, typename XPlanT<N / 2>::type,
-                         typename XPlanT<N / 2>::type>;
-
 // which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; body of length N
 template <typename R, int WHICH, int N> struct EnsBody;
 template <typename R, int N> struct EnsBody<R, 0, N> { using B = EZinv<R, N>; };
-// x-pass: the single-channel launch's engine for the length (register x-pass where enabled), 128 threads
-template <typename R, int N, bool VPL>
-using EX = typename std::conditional<xreg_enabled(N), XregBody<R, N, ENS_PLANE_NTH, VPL>, EXpass<R, N, VPL>>::type;
+// x-pass: the register x-pass body of the single-channel launches, 128 threads
+template <typename R, int N, bool VPL> using EX = XregBody<R, N, ENS_PLANE_NTH, VPL>;
 template <typename R, int N> struct EnsBody<R, 1, N> { using B = EX<R, N, false>; };
 template <typename R, int N> struct EnsBody<R, 2, N> { using B = EX<R, N, true>; };
 template <typename R, int N> struct EnsBody<R, 3, N> { using B = EZfwd<R, N, false>; };
@@ -96,10 +86,10 @@ static void ens_plane_grid(const PlaneArgs &a, int which, int &gx, int &gy, int 
   }
 }
 
-int ens_plane_rows_ok(int Mx, int Mz) {
-  const int rb = Mx == 48 ? EX<double, 48, false>::ROWS : Mx == 96 ? EX<double, 96, false>::ROWS
-                                                      : EX<double, 192, false>::ROWS;
-  return (xreg_enabled(Mx) || Mz % rb == 0) ? 1 : 0;  // the register x-pass takes any Mz
+int ens_plane_rows_ok(int Mx, int Mz) {  // the register x-pass takes any Mz
+  (void)Mx;
+  (void)Mz;
+  return 1;
 }
 
 int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, hipStream_t st) {

@@ -10,8 +10,8 @@
 //                 row and the previous one kept in registers, the y-interpolated row (interp_u/interp_v,
 //                 FOU3D.f90:373-418): the z-transform is linear, so interpolating after it equals
 //                 interpolating before it.
-//   xpass_kernel: I rows -> physical rows -> products -> P[plane][product][z][i]          (x-pass)
-//                 Also the CFL maxima of dtc_calc (FOU3D.f90:613-651) at kRK = 1.
+//   xreg_kernel : I rows -> physical rows -> products -> P[plane][product][z][i]          (x-pass,
+//                 xpass_reg.h); also the CFL maxima of dtc_calc (FOU3D.f90:613-651) at kRK = 1.
 //   zfwd_kernel : P -> forward z FFT -> der_x + der_z (masked) and raw fluxes in device layout.
 // Planes: u-plane j (u1, u2_itp, u3 -> uu, uw, vv, ww) and v-plane j (u1_itp, u2, u3_itp -> uv, wv) for
 // j = 1..nyu.  v-planes run to j = nyv+1 as the reference does on one rank (start.f90:1128-1180 gives
@@ -116,138 +116,6 @@ __global__ void __launch_bounds__(NTH, WPE) zinv_kernel(PlaneArgs a) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
   B::run(a, lb, gx, gy, lds);
-}
-
-// ------------------------------------------------------------------------------------------------
-// x-pass.  A real row x of length MX is packed as z_m = x_{2m} + i x_{2m+1} (H = MX/2 complex), so one
-// length-H complex FFT does each real transform.  Inverse (rft_buff.f:76-84 semantics: Im c_0 and
-// Im c_{MX/2} ignored, unnormalised Hermitian synthesis):
-//   Z_n = (X_n + conj X_{H-n}) + i e^{+2 pi i n/MX} (X_n - conj X_{H-n}),  z = IFFT_H(Z)
-// Forward (rft_buff.f:62-74: scaled 1/MX, Im c_0 = 0):
-//   X_k = [ (Z_k + conj Z_{H-k}) - i e^{-2 pi i k/MX} (Z_k - conj Z_{H-k}) ] / (2 MX),  Z = FFT_H(z)
-// VPL selects the plane type (the launch covers planes of one type).  The 3 velocities go to the row's
-// slots 0..2 and the products are formed in place: u-planes uu, uw, vv, ww in 4 slots per row, v-planes
-// uv, wv in slots 0..1 of 3 slots per row (the smaller LDS footprint keeps more v-rows resident).
-
-template <bool VPL> constexpr int xpass_slots() { return VPL ? 3 : 4; }  // LDS slots per row
-
-template <typename R, int MX, int RB, int NTH_, bool VPL, class PLI, class PLF, int PS = 4>
-struct XpassBody {
-  static constexpr int NTH = NTH_;
-  static constexpr int H = MX / 2;
-  static constexpr int NSL = xpass_slots<VPL>(), NP = VPL ? 2 : 4;
-  static constexpr int NS = RB * NSL;
-  static constexpr int ROWS = RB;  // rows per body
-  // plain thread indices: the x-pass's occupancy is set by its LDS, not by its registers; twiddles from
-  // the pass-blocked copy after twh (contiguous loads across the butterflies of a pass)
-  using LI = Lay<H, NS, RB * 3, RB, NSL, PS, true, true, false, true>;  // inverse: active f*RB + r -> slot r*NSL + f
-  using LF = Lay<H, NS, RB * NP, NP, NSL, PS, true, false, false, true>;  // forward: slots r*NSL + p, p < NP
-  static constexpr size_t LDS = sizeof(cx<R>) * LI::SIZE + sizeof(double) * (NTH / 64);
-  // grid (gx row tiles, planes x members); lb = bx + gx * by
-  __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
-};
-template <typename R, int MX, int RB, int NTH_, bool VPL, class PLI, class PLF, int PS>
-__device__ __forceinline__ void XpassBody<R, MX, RB, NTH_, VPL, PLI, PLF, PS>::run(const PlaneArgs &a_, int lb,
-                                                                                  int gx, int gy,
-                                                                                  unsigned char *lds) {
-  (void)gy;
-  const int bx = lb % gx, by = lb / gx;
-  const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
-  static_assert(std::is_same<PLI, typename XPlanT<H>::type>::value &&
-                    std::is_same<PLF, typename XPlanT<H>::type>::value,
-                "twh's pass-blocked twiddles are built for the plan XPlanT<H>");
-  cx<R> *A = reinterpret_cast<cx<R> *>(lds);
-  double *red = reinterpret_cast<double *>(lds + sizeof(cx<R>) * LI::SIZE);
-  const int jj = by % a.nbj, j = a.j0 + jj;
-  const int z0 = bx * RB;
-  constexpr int Kx = MX / 3 + 1;   // retained x-modes Nx/2 + 1, Nx = 2 MX/3 (the launcher checks a.Kx)
-  const int MZ = a.Mz;
-  const cx<R> *twx = reinterpret_cast<const cx<R> *>(a.twx);  // e^{-2 pi i m/MX}, m < MX
-  const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}, m < H, then its pass blocks
-  constexpr int Kp = row_pitch<R>(Kx);
-  const size_t fsz = (size_t)MZ * Kp;
-  const size_t pl = (size_t)((VPL ? a.nbj : 0) + jj);
-  const cx<R> *I = reinterpret_cast<const cx<R> *>(a.I) + pl * 3 * fsz;
-  cx<R> *P = reinterpret_cast<cx<R> *>(a.P) + pl * 4 * fsz;
-
-  // packed half-length spectrum Z_n of field f of row z0 + r (active transform q = f*RB + r)
-  auto ld = [&](int i, auto rM, int q) -> cx<R> {
-    const int n = i + decltype(rM)::value;
-    const int f = q / RB, r = q - f * RB;
-    const cx<R> *row = I + f * fsz + (size_t)(z0 + r) * Kp;
-    auto X = [&](int m) -> cx<R> { return row[m]; };
-    cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
-    if (n < Kx) xa = X(n);
-    if (n == 0) xa.y = 0;
-    if (n > 0 && H - n < Kx) xb = X(H - n);
-    const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
-    const cx<R> e = cconj(twx[n]);                        // e^{+2 pi i n/MX}
-    return cadd(sum, cmul_si<1>(cmul(dif, e)));
-  };
-  fft_from_mem<R, LI, NTH, +1, PLI>(A, twh, ld);
-
-  R m1 = 0, m3 = 0;
-  const bool cfl = (a.kRK == 1) && !VPL && (j <= a.nyu - 1);
-  // each thread stays on one row (NTH/RB threads per row) and strides its positions: no division
-  constexpr int TPR = NTH / RB;
-  static_assert(NTH % RB == 0, "threads per row");
-  cx<R> *const prow = A + ((int)threadIdx.x / TPR) * NSL * LI::LR;
-  for (int n = (int)threadIdx.x % TPR; n < H; n += TPR) {
-    const int e = LI::row(n);
-    cx<R> *b0 = prow + e;
-    const cx<R> zu = b0[0], zv = b0[LI::LR], zw = b0[2 * LI::LR];
-    if (!VPL) {
-      b0[0] = mk<R>(zu.x * zu.x, zu.y * zu.y);                // uu
-      b0[LI::LR] = mk<R>(zu.x * zw.x, zu.y * zw.y);           // uw
-      b0[2 * LI::LR] = mk<R>(zv.x * zv.x, zv.y * zv.y);       // vv
-      b0[3 * LI::LR] = mk<R>(zw.x * zw.x, zw.y * zw.y);       // ww
-      if (cfl) {
-        m1 = fmax(m1, fmax(fabs(zu.x), fabs(zu.y)));
-        m3 = fmax(m3, fmax(fabs(zw.x), fabs(zw.y)));
-      }
-    } else {
-      b0[0] = mk<R>(zu.x * zv.x, zu.y * zv.y);                // uv
-      b0[LI::LR] = mk<R>(zw.x * zv.x, zw.y * zv.y);           // wv
-    }
-  }
-  if (cfl) {
-    const double b1 = block_max<NTH>((double)m1, red);
-    const double b3 = block_max<NTH>((double)m3, red);
-    if (threadIdx.x == 0) {  // spread over CFL_SLOTS addresses: one address for every block serialises
-      const int slot = (bx + by) & (CFL_SLOTS - 1);
-      atomic_max_nonneg(&a.cfl[slot], b1);
-      atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);
-    }
-  }
-  __syncthreads();
-  fft_lds<R, LF, NTH, -1, PLF>(A, twh);
-
-  const R sc = (R)(0.5 / (double)MX);
-  {  // each thread stays on one (row, product) pair and strides its modes: no division per output
-    constexpr int SLOTS = RB * NP, TPS = NTH / SLOTS;
-    static_assert(TPS >= 1, "threads per output row");
-    const int q = (int)threadIdx.x / TPS, r = q / NP, p = q - r * NP;
-    if (q < SLOTS) {
-      const cx<R> *b0 = A + (r * NSL + p) * LF::LR;
-      cx<R> *out = P + ((size_t)p * MZ + (z0 + r)) * Kp;
-      for (int k = (int)threadIdx.x % TPS; k < Kx; k += TPS) {
-        const cx<R> zk = b0[LF::row(k)];
-        const cx<R> zm = cconj(b0[LF::row(k == 0 ? 0 : H - k)]);
-        const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
-        cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
-        x = cscale(x, sc);
-        if (k == 0) x.y = 0;
-        out[k] = x;
-      }
-    }
-  }
-}
-
-template <typename R, int MX, int RB, int NTH, bool VPL, class PLI, class PLF, int WPE = 1, int PS = 4>
-__global__ void __launch_bounds__(NTH, WPE) xpass_kernel(PlaneArgs a) {
-  using B = XpassBody<R, MX, RB, NTH, VPL, PLI, PLF, PS>;
-  __shared__ __align__(16) unsigned char lds[B::LDS];
-  B::run(a, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, lds);
 }
 
 // ------------------------------------------------------------------------------------------------

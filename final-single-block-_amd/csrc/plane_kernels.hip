@@ -165,25 +165,15 @@ static int launch_which(const PlaneArgs &a, int nw, bool fp32, int which, int v,
   return -1;
 }
 
-// The x-pass's twiddle table twh holds H = Mx/2 entries e^{-2 pi i m/H} followed by H more: for the LDS
-// x-pass their pass-blocked copy for its plan (pass_twiddles_kernel), for the register x-pass the lane
-// twiddles twr (xreg_twiddles_kernel); this fills the second half.
+// The x-pass's twiddle table twh holds H = Mx/2 entries e^{-2 pi i m/H} followed by H more, the register
+// x-pass's lane twiddles twr (xreg_twiddles_kernel); this fills the second half.
 int build_xpass_twiddles(void *twh, int Mx, bool fp32, hipStream_t st) {
 #define CASE(N)                                                                                          \
-  if (Mx == N && xreg_enabled(N)) {                                                                      \
+  if (Mx == N) {                                                                                         \
     if (fp32)                                                                                            \
       xreg_twiddles_kernel<float, N / 2><<<(N / 2 + 255) / 256, 256, 0, st>>>(static_cast<cx<float> *>(twh)); \
     else                                                                                                 \
       xreg_twiddles_kernel<double, N / 2><<<(N / 2 + 255) / 256, 256, 0, st>>>(static_cast<cx<double> *>(twh)); \
-    return 0;                                                                                            \
-  }                                                                                                      \
-  if (Mx == N) {                                                                                         \
-    if (fp32)                                                                                            \
-      pass_twiddles_kernel<float, N / 2, typename XPlanT<N / 2>::type><<<(N / 2 + 255) / 256, 256, 0, st>>>( \
-          static_cast<cx<float> *>(twh));                                                                \
-    else                                                                                                 \
-      pass_twiddles_kernel<double, N / 2, typename XPlanT<N / 2>::type><<<(N / 2 + 255) / 256, 256, 0, st>>>( \
-          static_cast<cx<double> *>(twh));                                                               \
     return 0;                                                                                            \
   }
   DNS_SIZES(CASE)

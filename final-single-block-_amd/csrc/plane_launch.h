@@ -13,8 +13,8 @@ namespace dnsk {
 // picks among them.  Plans and tiles of the tuned lengths were chosen by timing on MI355X:
 //   z-transforms of 576: 12.12.4 (one butterfly per thread in every pass with 48 threads per column),
 //     16 columns per workgroup (256-byte row segments in FP64);
-//   x-pass at 576 and 1152 points: the register x-pass (xpass_reg.h), 9 points per lane and 32 or 64
-//     lanes per row; other lengths: the LDS x-pass with the plans of XPlanT.
+//   x-pass: the register x-pass (xpass_reg.h) at every length, 3 or 9 points per lane (9 at 576 and
+//     1152, with 32 or 64 lanes per row).
 #define DNS_SIZES(X) X(12) X(24) X(48) X(96) X(144) X(192) X(288) X(384) X(576) X(1152)
 #define DNS_TUNED(N) ((N) == 192 || (N) == 576 || (N) == 1152)
 
@@ -36,20 +36,6 @@ template <int N, int V> struct ZTile {
   static constexpr int NTH = p12 ? (48 * T < 768 ? 48 * T : 768) : (NB < 64 ? 64 : NB > 256 ? 256 : NB);
   static constexpr int ZCH = (N == 576 && V == 1) ? 4 : (N == 576 && V == 3) ? 8 : 2;
 };
-// LDS x-pass tiles (RB rows, NTH threads).  576 points: 4, 2 or 1 rows per workgroup with one wavefront
-// per row.  1152 points: one row per workgroup with 2, 4 or 1 wavefronts.  (Both lengths run the register
-// x-pass, xreg_v; the ensemble's grouped launch keeps these tiles for its lengths.)  Shorter rows have too
-// few butterflies per pass for a wavefront, so RPW rows share one: workgroups of 2, 1 or 4 wavefronts.
-template <int MX, int V> struct XTile {
-  static constexpr int H = MX / 2;
-  static constexpr int RPW = H <= 48 ? 4 : H <= 96 ? 2 : 1;   // rows per wavefront below 576 points
-  static constexpr bool big = MX >= 1152, mid = MX == 576;
-  static constexpr int RB = big ? 1 : mid ? (V == 0 ? 4 : V == 1 ? 2 : 1)
-                                         : (V == 0 ? 2 * RPW : V == 1 ? RPW : 4 * RPW);
-  static constexpr int NTH = big ? (V == 0 ? 128 : V == 1 ? 256 : 64) : mid ? (V == 0 ? 256 : V == 1 ? 128 : 64)
-                                                                          : (V == 0 ? 128 : V == 1 ? 64 : 256);
-};
-
 template <typename R, int MZ, int V>
 inline void zinv_v(const PlaneArgs &a, int nw, hipStream_t st) {
   using Z = ZTile<MZ, V>;
@@ -81,21 +67,7 @@ inline void xreg_v(const PlaneArgs &a, int nw, hipStream_t st) {
 
 template <typename R, int MX, int V, bool VPL>
 inline void xpass_v(const PlaneArgs &a, int nw, hipStream_t st) {
-  using X = XTile<MX, V>;
-  constexpr int H = MX / 2;
-  if constexpr (xreg_enabled(MX)) {
-    xreg_v<R, MX, V, VPL>(a, nw, st);
-  } else if constexpr (V > 0 && (size_t)X::RB * xpass_slots<VPL>() * (H + (H >> 4)) * sizeof(cx<R>) > LDS_MAX - 1024) {
-    xpass_v<R, MX, 0, VPL>(a, nw, st);
-  } else {
-    using PL = typename XPlanT<H>::type;
-    if (a.Mz % X::RB != 0) {  // rows that do not fill the tile: one row per single-wavefront workgroup
-      xpass_kernel<R, MX, 1, 64, VPL, PL, PL, 1, 4><<<dim3(a.Mz, nw * a.nm), 64, 0, st>>>(a);
-      return;
-    }
-    dim3 g(a.Mz / X::RB, nw * a.nm);
-    xpass_kernel<R, MX, X::RB, X::NTH, VPL, PL, PL, 1, 4><<<g, X::NTH, 0, st>>>(a);
-  }
+  xreg_v<R, MX, V, VPL>(a, nw, st);
 }
 
 template <typename R, int N>

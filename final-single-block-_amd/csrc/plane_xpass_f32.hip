@@ -1,4 +1,4 @@
-// x-pass kernels (xpass_kernel) of every supported length, both plane kinds, in FP32.
+// x-pass kernels (xreg_kernel) of every supported length, both plane kinds, in FP32.
 #include "plane_launch.h"
 
 namespace dnsk {

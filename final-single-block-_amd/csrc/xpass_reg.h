@@ -1,6 +1,6 @@
-// Register-resident x-pass (the x-direction half of four_to_phys_u / phys_to_four_du and the products
-// of ops_in_planes2, FOU3D.f90:1514-1652, rft_buff.f:62-84) for rows whose packed length H = MX/2 is
-// RR * L with RR = 1, 3 or 9 and L = 2..64 a power of two.
+// The x-pass (the x-direction half of four_to_phys_u / phys_to_four_du and the products of
+// ops_in_planes2, FOU3D.f90:1514-1652, rft_buff.f:62-84), register-resident, for rows whose packed
+// length H = MX/2 is RR * L with RR = 1, 3 or 9 and L = 2..64 a power of two (every supported Mx).
 //
 // One row transform per group of L lanes, RR points per lane, no LDS in the transforms:
 //   inverse (sign +1), input Z[L n1 + t] in lane t, register n1 (the row's memory order, so the loads
@@ -14,8 +14,12 @@
 // its own registers.  The only cross-lane traffic besides the butterflies is the Hermitian partners
 // (X[H - n] of the inverse's pre-processing, Z[H - k] of the forward's post-processing), in lane
 // (L - t) mod L (ds_bpermute).
-// Pre- and post-processing are the packed real transform of the LDS x-pass (same formulas, same order
-// of operations); the transforms themselves round differently from the LDS Stockham engine.
+// Pre- and post-processing are the packed real transform (rft_buff.f's semantics): inverse
+//   Z_n = (X_n + conj X_{H-n}) + i e^{+2 pi i n/MX} (X_n - conj X_{H-n}),  z = IFFT_H(Z)
+// with Im c_0 and Im c_{MX/2} ignored (rft_buff.f:76-84), and forward
+//   X_k = [ (Z_k + conj Z_{H-k}) - i e^{-2 pi i k/MX} (Z_k - conj Z_{H-k}) ] / (2 MX),  Z = FFT_H(z)
+// scaled 1/MX with Im c_0 = 0 (rft_buff.f:62-74); z packs a real row x as z_m = x_{2m} + i x_{2m+1}.
+// The products are formed pointwise: u-planes uu, uw, vv, ww (P slots 0..3), v-planes uv, wv (0..1).
 #pragma once
 #include "plane_kernels.h"
 
@@ -26,11 +30,6 @@ template <int H> struct XregShape {
   static constexpr int RR = H / L;
   static constexpr bool ok = L >= 2 && L <= 64 && (RR == 1 || RR == 3 || RR == 9);
 };
-// x lengths that use the register x-pass (all of its tile variants, so every variant gives the same bits)
-__host__ __device__ constexpr bool xreg_enabled(int MX) {
-  return MX == 48 || MX == 96 || MX == 192 || MX == 576 || MX == 1152;
-}
-
 // value of lane (lane ^ M) of the wavefront, M < 64 a power of two
 template <int M> __device__ __forceinline__ int lane_xor_i(int v) {
   if constexpr (M == 1) {
