@@ -113,6 +113,17 @@ def ensemble_leg(steps, warmup, prec, eager=False):
     ens.step(2)
     ens.sync()
     tim = ens.timing()
+    ens.set_timing(0)
+    # the same members in one persistent launch per substep (task queue over every member and class)
+    reupload()
+    ens.set_launch(1)
+    _, nwg, ntask = ens.launch()
+    ens_step(max(warmup, 1))
+    ens.sync()
+    t0 = time.perf_counter()
+    ens_step(steps)
+    ens.sync()
+    sec_persist = (time.perf_counter() - t0) / steps
     ens.close()
     # the same members as three handles on three streams, each autotuned, each step from its own graph
     for g in handles:
@@ -136,6 +147,9 @@ def ensemble_leg(steps, warmup, prec, eager=False):
                       + ("eager steps" if eager else "step replayed from a hipGraph"),
             "kernels_ms_per_substep": {k: v for k, v in tim.items() if k != "n"},
             "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
+            "persistent": {"value": 1.0 / sec_persist, "ms_per_step": sec_persist * 1e3,
+                           "launch": "one persistent launch per substep: %d workgroups over a queue of "
+                                     "%d/%d/%d tasks (substeps 1/2/3)" % (nwg, *ntask)},
             "three_streams": {"value": 1.0 / sec_streams, "ms_per_step": sec_streams * 1e3,
                               "launch": "one handle per size on its own stream, autotuned, own hipGraph"}}
 

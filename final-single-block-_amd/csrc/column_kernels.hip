@@ -179,81 +179,10 @@ __global__ void meanflow_kernel(ColArgs a_) {
   for (int j = 0; j < n; j++) a.u1[(size_t)j * C].x = a.u1[(size_t)j * C].x + g * u11[j];
 }
 
-// meanflow_ctU for the one-pass column stage, which has already run solveP and v_corr.  For mode (0,0)
-// (kx = kz = 0) neither the divergence nor v_corr's interior update reads u1 (both multiply it by kx = 0),
-// so the correction commutes with them: the interior rows get + g u11 exactly as flowrate_corr adds it
-// (littleharsh.f90:515-548), and the ghost rows are then v_corr's wall rows of the corrected interior
-// (littleharsh.f90:935-943), which overwrite what flowrate_corr added there.  The block stages the
-// matrix rows, yv and u1's mode (0,0) in LDS; one thread runs LUsol0's sweeps in the reference's order
-// (tridLU_3D.f90:134-160, 299-325) and the two flow-rate sums.
-constexpr int MEANFLOW_MAXN = 516;  // nyu + 2 rows, nyu <= 512 (the one-pass column stage's range)
-__device__ __forceinline__ void meanflow_post_body(const ColArgs &a_, int member) {
-  const ColArgs a = col_view(a_, member);
-  __shared__ double c1[MEANFLOW_MAXN], c2[MEANFLOW_MAXN], c3[MEANFLOW_MAXN], yv[MEANFLOW_MAXN],
-      ur[MEANFLOW_MAXN], u11[MEANFLOW_MAXN], w3[MEANFLOW_MAXN];
-  __shared__ double gsh;
-  const int n = a.nyu + 2, C = a.C;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    if (j >= 1 && j < n - 1) {
-      c1[j] = D2U(1, j);
-      c2[j] = D2U(2, j);
-      c3[j] = D2U(3, j);
-    }
-    if (j <= a.nyv + 1) yv[j] = a.yv[j];
-    ur[j] = a.u1[(size_t)j * C].x;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double dt = a.st->dt;
-    const double beta = a.bRK * dt / a.Re;
-    const double bdt = -a.dRK * dt;
-    double w2 = 1.0 / 1.0, y = 0.0 * w2, w3p = a.gw1 * w2;
-    u11[0] = y;
-    w3[0] = w3p;
-    for (int j = 1; j < n; j++) {
-      double a1, a2, a3, r;
-      if (j < n - 1) {
-        a1 = -beta * c1[j];
-        a2 = 1.0 - beta * c2[j];
-        a3 = -beta * c3[j];
-        r = bdt;
-      } else {
-        a1 = a.gw2; a2 = 1.0; a3 = 0.0; r = 0.0;
-      }
-      w2 = 1.0 / (a2 - a1 * w3p);
-      y = (r - a1 * y) * w2;
-      w3p = a3 * w2;
-      u11[j] = y;
-      w3[j] = w3p;
-    }
-    double x = y;
-    for (int j = n - 2; j >= 0; j--) {
-      x = u11[j] - w3[j] * x;
-      u11[j] = x;
-    }
-    double Qx1 = 0.0, Qx0 = 0.0;
-    // flowrateRe's dummy u(1:nn+2) starts at u11(0) (littleharsh.f90:505-511 with start.f90:261)
-    for (int j = 1; j <= a.nyv + 1; j++) Qx1 = Qx1 + (yv[j] - yv[j - 1]) * u11[j - 1];
-    for (int j = 1; j <= a.nyv + 1; j++) Qx0 = Qx0 + (yv[j] - yv[j - 1]) * ur[j];
-    const double g = (a.st->QxT - Qx0) / Qx1;
-    a.st->mpgx = a.st->mpgx + g;
-    a.st->dgx = g;
-    gsh = g;
-  }
-  __syncthreads();
-  const double g = gsh;
-  for (int j = 1 + threadIdx.x; j <= a.nyu; j += blockDim.x) {
-    const double v = ur[j] + g * u11[j];
-    ur[j] = v;
-    a.u1[(size_t)j * C].x = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {  // v_corr's wall rows of the corrected interior
-    a.u1[0] = rmul(a.gbc, d2(ur[1], a.u1[C].y));
-    a.u1[(size_t)(a.nyu + 1) * C] = rmul(a.gbc, d2(ur[a.nyu], a.u1[(size_t)a.nyu * C].y));
-  }
+__global__ void __launch_bounds__(256) meanflow_post_kernel(ColArgs a) {
+  __shared__ MeanflowLds L;
+  meanflow_post_body(a, blockIdx.y, L);
 }
-__global__ void __launch_bounds__(256) meanflow_post_kernel(ColArgs a) { meanflow_post_body(a, blockIdx.y); }
 
 // one block per member of the groups with flags[g] = 1 (constant flow rate)
 __global__ void __launch_bounds__(256) ens_meanflow_post_kernel(EnsColLaunch E, int4 flags) {
@@ -264,7 +193,8 @@ __global__ void __launch_bounds__(256) ens_meanflow_post_kernel(EnsColLaunch E, 
     if (m < n) break;
     m -= n;
   }
-  if (g < E.ng) meanflow_post_body(E.a[g], m);
+  __shared__ MeanflowLds L;
+  if (g < E.ng) meanflow_post_body(E.a[g], m, L);
 }
 void launch_ens_meanflow_post(const EnsColLaunch &E, const int *flags, hipStream_t st) {
   static_assert(ENS_MAXG == 4, "flags travel as an int4");

@@ -833,6 +833,13 @@ struct dns_ensemble {
   hipEvent_t ev[5] = {};
   double tsum[7] = {};
   int tcount = 0;
+  // persistent launch (dns_ensemble_set_launch): one kernel per substep over a task queue
+  int launch = 0;
+  dnsk::PersistArgs *pargs = nullptr;  // device: the arguments of substeps kRK = 1, 2, 3
+  dnsk::PTask *ptask = nullptr;        // device: the three queues one after another
+  unsigned *pctr = nullptr;            // device: queue head, finished workgroups, timeout flag, counters
+  int nwg = 0;                         // workgroups of a persistent launch
+  int ntask[3] = {};
 };
 
 static void ens_mark(dns_ensemble *e, int q) {
@@ -864,6 +871,13 @@ static int ens_substep(dns_ensemble *e, int kRK) {
     Cl.a[g] = col_args(h, kRK);
     flags[g] = h->prm.flag_ctpress == 0;
   }
+  if (e->launch == 1) {  // everything of the substep in one launch
+    ens_mark(e, 0);
+    dnsk::launch_ens_persist(e->pargs + (kRK - 1), e->fp32, e->nwg, e->stream);
+    for (int q = 1; q <= 4; q++) ens_mark(e, q);
+    HIPCHK(hipGetLastError());
+    return DNS_OK;
+  }
   ens_mark(e, 0);
   dnsk::launch_ens_reset(S, kRK, e->stream);
   if (kRK == 1) dnsk::launch_ens_ghost(S, e->stream);
@@ -893,8 +907,19 @@ static void ens_accumulate_timing(dns_ensemble *e) {
   e->tcount++;
 }
 
+static void ens_free_persist(dns_ensemble *e) {
+  if (e->pargs) (void)hipFree(e->pargs);
+  if (e->ptask) (void)hipFree(e->ptask);
+  if (e->pctr) (void)hipFree(e->pctr);
+  e->pargs = nullptr;
+  e->ptask = nullptr;
+  e->pctr = nullptr;
+  e->launch = 0;
+}
+
 static void ens_destroy_all(dns_ensemble *e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_free_persist(e);
   if (e->graph) (void)hipGraphExecDestroy(e->graph);
   if (e->done) (void)hipEventDestroy(e->done);
   for (int q = 0; q < 5; q++)
@@ -1001,6 +1026,107 @@ extern "C" int dns_ensemble_step_graph(dns_ensemble *e, int nsteps) {
 extern "C" int dns_ensemble_sync(dns_ensemble *e) {
   if (!e) return fail(DNS_ERR_ARG, "dns_ensemble_sync: null ensemble");
   HIPCHK(hipStreamSynchronize(e->stream));
+  if (e->launch == 1) {  // a dependency wait of the persistent launch that ran out (results not valid)
+    unsigned err = 0;
+    HIPCHK(hipMemcpy(&err, e->pctr + dnsk::PCTR_ERR, sizeof err, hipMemcpyDeviceToHost));
+    if (err) {
+      HIPCHK(hipMemset(e->pctr + dnsk::PCTR_ERR, 0, sizeof err));
+      return fail(DNS_ERR_HIP, "dns_ensemble_sync: a task of the persistent launch waited too long for the "
+                  "tasks it depends on; the fields of these steps are not valid");
+    }
+  }
+  return DNS_OK;
+}
+
+// Tasks per queue entry by class (PC_RESET .. PC_MF): a task covers that many units (two plane blocks, or
+// one column-pass workgroup) of one member.
+static const int PERSIST_UNITS[dnsk::PC_N] = {1, 1, 16, 32, 32, 1, 16, 16, 8, 1};
+
+extern "C" int dns_ensemble_set_launch(dns_ensemble *e, int mode, const int *units) {
+  if (!e) return fail(DNS_ERR_ARG, "dns_ensemble_set_launch: null ensemble");
+  if (mode != 0 && mode != 1) return fail(DNS_ERR_ARG, "dns_ensemble_set_launch: mode %d (0 or 1)", mode);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (e->graph) {  // the captured step launches the kernels of the previous mode
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+  ens_free_persist(e);
+  if (mode == 0) return DNS_OK;
+  int un[dnsk::PC_N];
+  for (int c = 0; c < dnsk::PC_N; c++) {
+    un[c] = units && units[c] > 0 ? units[c] : PERSIST_UNITS[c];
+    if (un[c] > 1 << 20) return fail(DNS_ERR_ARG, "dns_ensemble_set_launch: %d units per task", un[c]);
+  }
+  const int ng = (int)e->h.size();
+  dnsk::PersistGroup q[dnsk::ENS_MAXG];
+  int nm[dnsk::ENS_MAXG];
+  for (int g = 0; g < ng; g++) {
+    dns_handle *h = e->h[g];
+    PlaneArgs a = plane_args(h, 1);
+    a.j0 = 1;
+    a.nbj = h->nyu;
+    nm[g] = h->nm;
+    if (dnsk::ens_persist_group(a, h->prm.flag_ctpress == 0, q[g]))
+      return fail(DNS_ERR_ARG, "dns_ensemble_set_launch: group %d (%d x %d points, ny = %d) is not supported by the "
+                  "persistent launch (each z-inverse chunk of planes must be whole: ny - 1 even)", g, h->Mx,
+                  h->Mz, h->ny);
+  }
+  std::vector<dnsk::PTask> all, tk;
+  int nctr = 0;
+  dnsk::PersistArgs A[3];
+  for (int k = 0; k < 3; k++) {
+    if (dnsk::ens_persist_tasks(q, nm, ng, k + 1, un, tk, nctr))
+      return fail(DNS_ERR_ARG, "dns_ensemble_set_launch: the task queue of substep %d does not order its tasks", k + 1);
+    e->ntask[k] = (int)tk.size();
+    memset(&A[k], 0, sizeof A[k]);
+    A[k].ntask = (int)tk.size();
+    A[k].nctr = nctr;
+    A[k].kRK = k + 1;
+    A[k].ng = ng;
+    for (int g = 0; g < ng; g++) {
+      dns_handle *h = e->h[g];
+      A[k].pa[g] = plane_args(h, k + 1);
+      A[k].pa[g].j0 = 1;
+      A[k].pa[g].nbj = h->nyu;
+      A[k].ca[g] = col_args(h, k + 1);
+      A[k].q[g] = q[g];
+    }
+    all.insert(all.end(), tk.begin(), tk.end());
+  }
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int per = dnsk::ens_persist_max_blocks(e->fp32);
+  if (per < 1 || ncu < 1) return fail(DNS_ERR_HIP, "dns_ensemble_set_launch: the persistent kernel does not fit a CU");
+  e->nwg = per * ncu;
+  hipError_t r = hipMalloc((void **)&e->pargs, sizeof A);
+  if (r == hipSuccess) r = hipMalloc((void **)&e->ptask, all.size() * sizeof(dnsk::PTask));
+  if (r == hipSuccess) r = hipMalloc((void **)&e->pctr, (dnsk::PCTR0 + (size_t)nctr) * sizeof(unsigned));
+  if (r == hipSuccess) r = hipMemcpy(e->ptask, all.data(), all.size() * sizeof(dnsk::PTask), hipMemcpyHostToDevice);
+  if (r == hipSuccess) {
+    size_t off = 0;
+    for (int k = 0; k < 3; k++) {
+      A[k].task = e->ptask + off;
+      A[k].ctr = e->pctr;
+      off += (size_t)e->ntask[k];
+    }
+    r = hipMemcpy(e->pargs, A, sizeof A, hipMemcpyHostToDevice);
+  }
+  if (r == hipSuccess) r = hipMemset(e->pctr, 0, (dnsk::PCTR0 + (size_t)nctr) * sizeof(unsigned));
+  if (r != hipSuccess) {
+    ens_free_persist(e);
+    return fail(DNS_ERR_HIP, "dns_ensemble_set_launch: %s", hipGetErrorString(r));
+  }
+  e->launch = 1;
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_get_launch(dns_ensemble *e, int *mode, int *nwg, int *ntask) {
+  if (!e || !mode) return fail(DNS_ERR_ARG, "dns_ensemble_get_launch: null argument");
+  *mode = e->launch;
+  if (nwg) *nwg = e->launch ? e->nwg : 0;
+  if (ntask)
+    for (int k = 0; k < 3; k++) ntask[k] = e->launch ? e->ntask[k] : 0;
   return DNS_OK;
 }
 

@@ -264,7 +264,7 @@ struct Lay {
   __device__ __forceinline__ static int slot(int a) {
     return TR_ ? (a % GA_) * GS_ + a / GA_ : (a / GA_) * GS_ + a % GA_;
   }
-  __device__ __forceinline__ static int tid();
+  template <int NTH> __device__ __forceinline__ static int tid();
   // thread work item b of a pass with M butterflies per transform -> (active transform a, butterfly i)
   template <int M>
   __device__ __forceinline__ static void split(int b, int &a, int &i) {
@@ -281,17 +281,26 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Thread index within a body of NTH threads.  A workgroup may run several bodies of a power-of-two
+// NTH side by side (the ensemble's persistent launch), body b on threads b*NTH .. b*NTH+NTH-1; in a
+// launch whose workgroup is one body the mask is known to be a no-op (the launch bounds give
+// threadIdx.x < NTH) and disappears.
+template <int NTH> __device__ __forceinline__ int body_tid() {
+  return (NTH & (NTH - 1)) == 0 ? (int)(threadIdx.x & (NTH - 1)) : (int)threadIdx.x;
+}
+
 // Thread index, opaque to the optimiser: index arithmetic derived from it is recomputed where it is used
 // instead of being hoisted out of outer loops and kept live in registers across them.
-__device__ __forceinline__ int tidx() {
-  int t = threadIdx.x;
+template <int NTH> __device__ __forceinline__ int tidx() {
+  int t = body_tid<NTH>();
   asm volatile("" : "+v"(t));
   return t;
 }
 
 template <int N_, int NTR_, int NACT_, int GA_, int GS_, int PS_, bool BF_, bool TR_, bool OPQ_>
+template <int NTH>
 __device__ __forceinline__ int Lay<N_, NTR_, NACT_, GA_, GS_, PS_, BF_, TR_, OPQ_>::tid() {
-  return OPQ_ ? tidx() : (int)threadIdx.x;
+  return OPQ_ ? tidx<NTH>() : body_tid<NTH>();
 }
 
 // Butterfly of one pass on registers u[0..RAD): twiddle (P > 1), DFT.
@@ -317,7 +326,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   cx<R> u[PER][RAD];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = L::tid() + q * NTH;
+    const int b = L::template tid<NTH>() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -329,7 +338,7 @@ __device__ __forceinline__ void fft_pass(cx<R> *s, const cx<R> *__restrict__ tw)
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = L::tid() + q * NTH;
+    const int b = L::template tid<NTH>() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -414,7 +423,7 @@ __device__ __forceinline__ void fft_first(cx<R> *s, LD ld) {
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = L::tid() + q * NTH;
+    const int b = L::template tid<NTH>() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<M>(b, a, i);
@@ -443,7 +452,7 @@ __device__ __forceinline__ void fft_last(const cx<R> *s, const cx<R> *__restrict
   constexpr int PER = (NB + NTH - 1) / NTH;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int b = L::tid() + q * NTH;
+    const int b = L::template tid<NTH>() + q * NTH;
     if ((NB % NTH) == 0 || b < NB) {
       int a, i;
       L::template split<P>(b, a, i);

@@ -3,6 +3,8 @@
 #pragma once
 #include "dns_device.h"
 
+#include <vector>
+
 namespace dnsk {
 
 // Device layout of every spectral field: plane-major, row j of the field at [row][column] with
@@ -251,6 +253,47 @@ bool ens_column_rows_supported(int nyu);
 int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st);
 // meanflow_ctU after the column pass for the groups with a constant flow rate (flags[g] = 1)
 void launch_ens_meanflow_post(const EnsColLaunch &E, const int *flags, hipStream_t st);
+
+// ---- persistent launch of the ensemble (ensemble.hip): one launch per RK3 substep for every member of
+// every group.  Its workgroups take tasks from a queue in device memory; a task is a run of logical
+// blocks of one kernel class for one member, and waits for the tasks it depends on through per-member
+// completion counters.
+constexpr int PERSIST_NTH = 256;   // threads per workgroup: two 128-thread plane bodies side by side
+// task classes, in queue order
+enum : int { PC_RESET = 0, PC_GHOST, PC_ZINV, PC_XU, PC_XV, PC_CFLR, PC_ZU, PC_ZV, PC_COL, PC_MF, PC_N };
+// Units u0..u1-1 of class cls for member m of group g (plane classes: a unit is two logical blocks of
+// the member's grid, one per body; column stage: one workgroup of the group's column-pass shape); it may
+// start once counter dep[i] has reached need[i] (dep[i] < 0: none; the dependencies fill dep[] from 0) and
+// adds one to counter done; rel = 1 when a later task of the launch waits on that counter.
+struct PTask {
+  int cls, g, m, u0, u1, done;
+  int dep[3], need[3];
+  int rel;
+};
+// ctr[]: queue head, workgroups finished, dependency-wait timeout flag, then the completion counters
+constexpr int PCTR_HEAD = 0, PCTR_EXIT = 1, PCTR_ERR = 2, PCTR0 = 4;
+struct PersistGroup {
+  int gxz, gxx, gxf;   // tiles per row of the z-inverse, x-pass and z-forward grids
+  int nbz, nbx, nbf;   // their logical blocks per member
+  int ncb, cshape;     // column blocks per member, column-pass shape (rows bucket 32, 64, 96, 128, 160)
+  int ctp0;            // 1: constant flow rate (meanflow_ctU after the column pass)
+};
+struct PersistArgs {
+  PlaneArgs pa[ENS_MAXG];
+  ColArgs ca[ENS_MAXG];
+  PersistGroup q[ENS_MAXG];
+  const PTask *task;
+  unsigned *ctr;
+  int ntask, nctr, kRK, ng;   // tasks, completion counters
+};
+// Grids of group g (its PlaneArgs with j0 = 1, nbj = nyu, and its column rows); -1 when not supported.
+int ens_persist_group(const PlaneArgs &a, int ctp0, PersistGroup &q);
+// The queue of one substep: tasks in order and the number of completion counters; every dependency is
+// checked to be met by earlier tasks (so the queue cannot deadlock).  units[c]: units per task of class c.
+int ens_persist_tasks(const PersistGroup *q, const int *nm, int ng, int kRK, const int *units,
+                      std::vector<PTask> &tasks, int &ncounters);
+int launch_ens_persist(const PersistArgs *E, bool fp32, int nwg, hipStream_t st);  // E in device memory
+int ens_persist_max_blocks(bool fp32);   // workgroups of the persistent kernel resident per CU
 
 // layout
 void launch_ref_to_dev(const double2 *ref, double2 *dev, int C, int nrows, hipStream_t st);

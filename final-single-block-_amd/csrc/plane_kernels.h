@@ -166,7 +166,7 @@ __device__ __forceinline__ void ZfwdBody<R, MZ, T, NTH_, VPL, PL, PS>::run(const
     fft_from_mem<R, L, NTH, -1, PL>(s, tw, ld);
 #pragma unroll
     for (int q = 0; q < PERMAX; q++) {
-      const int idx = threadIdx.x + q * NTH;
+      const int idx = body_tid<NTH>() + q * NTH;
       if (idx < Nz * T) {
         const int k = idx / T, t = idx - k * T, i = i0 + t;
         const int zp = k < Nz / 2 ? k : k + MZ - Nz;

@@ -96,7 +96,8 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   const int bx = lb % gx, by = lb / gx;
   const PlaneArgs a = plane_view<R>(a_, by / a_.nbj);  // grid y = planes x ensemble members
   const int jj = by % a.nbj, j = a.j0 + jj;
-  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  const int ltid = body_tid<NTH>();  // this body's thread index (a workgroup may run several bodies)
+  const int lane = ltid & 63, wv = ltid >> 6;
   const int t = lane & (L - 1), g = lane / L;
   const int MZ = a.Mz;
   // this lane group's row; groups past the last row (Mz not a multiple of RB) redo the last row and store
@@ -254,7 +255,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
 #pragma unroll
     for (int i = 0; i < RR; i++) {
       if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
-      stash[i * NTH + threadIdx.x] = A[i];
+      stash[i * NTH + ltid] = A[i];
     }
     load_inverse(2, B);  // w
     inverse(B);
@@ -269,14 +270,14 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
     forward(B);
     store(3, B);
 #pragma unroll
-    for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + threadIdx.x];
+    for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + ltid];
     sq(A);  // uu
     forward(A);
     store(0, A);
-    if (a.kRK == 1 && j <= a.nyu - 1) {  // uniform over the workgroup
-      const double b1 = block_max<NTH>((double)m1, red);
-      const double b3 = block_max<NTH>((double)m3, red);
-      if (threadIdx.x == 0) {  // spread over CFL_SLOTS addresses: one address for every block serialises
+    if (a.kRK == 1) {  // uniform over the workgroup (m1 = m3 = 0 on planes past nyu - 1)
+      const double b1 = block_max_sub<NTH>((double)m1, red, ltid);
+      const double b3 = block_max_sub<NTH>((double)m3, red, ltid);
+      if (ltid == 0 && cfl) {  // spread over CFL_SLOTS addresses: one address for every block serialises
         const int slot = (bx + by) & (CFL_SLOTS - 1);
         atomic_max_nonneg(&a.cfl[slot], b1);
         atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);

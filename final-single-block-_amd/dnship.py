@@ -49,7 +49,7 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_refresh_stats_pressure", "dns_four_to_phys", "dns_four_to_phys_noalloc", "dns_ensemble_create",
            "dns_ensemble_destroy",
            "dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_sync", "dns_ensemble_set_timing",
-           "dns_ensemble_get_timing"]
+           "dns_ensemble_get_timing", "dns_ensemble_set_launch", "dns_ensemble_get_launch"]
 
 
 def lib():
@@ -106,6 +106,9 @@ def lib():
     for n in ("dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_set_timing"):
         getattr(L, n).argtypes = [vp, ctypes.c_int]
     L.dns_ensemble_get_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.dns_ensemble_set_launch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.dns_ensemble_get_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -364,8 +367,9 @@ class DNS:
 
 class Ensemble:
     """Several DNS handles (one per size, each with its members) advanced together: one grouped launch per
-    kernel class per RK3 substep for every member (dns_ensemble_*).  The handles stay usable for uploads,
-    downloads and scalars, and must stay open while the ensemble is."""
+    kernel class per RK3 substep for every member, or one persistent launch per substep (set_launch)
+    (dns_ensemble_*).  The handles stay usable for uploads, downloads and scalars, and must stay open while
+    the ensemble is."""
 
     def __init__(self, handles):
         self.handles = list(handles)
@@ -378,6 +382,24 @@ class Ensemble:
     def step_graph(self, n=1): _chk(lib().dns_ensemble_step_graph(self.e, n))
     def sync(self): _chk(lib().dns_ensemble_sync(self.e))
     def set_timing(self, on): _chk(lib().dns_ensemble_set_timing(self.e, 1 if on else 0))
+
+    # task classes of the persistent launch, in queue order (units per task: dns_ensemble_set_launch)
+    TASK_CLASSES = ("reset", "ghost", "zinv", "xpass_u", "xpass_v", "cfl_reduce", "zfwd_u", "zfwd_v",
+                    "col_pass", "meanflow")
+
+    def set_launch(self, mode, units=None):
+        """mode 0: grouped launches (one per kernel class per substep); 1: one persistent launch per
+        substep over a task queue.  units: {class name: units per task} (others default)."""
+        arr = None
+        if units:
+            arr = (ctypes.c_int * len(self.TASK_CLASSES))(*[int(units.get(c, 0)) for c in self.TASK_CLASSES])
+        _chk(lib().dns_ensemble_set_launch(self.e, int(mode), arr))
+
+    def launch(self):
+        """(mode, workgroups of a persistent launch, tasks of substeps 1..3)"""
+        mode, nwg, nt = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 3)()
+        _chk(lib().dns_ensemble_get_launch(self.e, ctypes.byref(mode), ctypes.byref(nwg), nt))
+        return mode.value, nwg.value, list(nt)
 
     def timing(self):
         t = np.zeros(7)

@@ -236,6 +236,19 @@ module dns_hip
       real(c_double), intent(out) :: t(7)
       integer(c_int), intent(out) :: n
     end function
+    ! mode 0 grouped launches, 1 one persistent launch per substep; units: c_null_ptr or c_loc of 10 ints
+    integer(c_int) function dns_ensemble_set_launch(e, mode, units) bind(C, name='dns_ensemble_set_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: mode
+      type(c_ptr), value :: units
+    end function
+    integer(c_int) function dns_ensemble_get_launch(e, mode, nwg, ntask) bind(C, name='dns_ensemble_get_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: mode, nwg
+      integer(c_int), intent(out) :: ntask(3)
+    end function
     ! device pointers (type(c_ptr)) and a stream (c_null_ptr: the null stream); no device allocation
     integer(c_int) function dns_four_to_phys_noalloc(Nx, Nz, Mx, Mz, nplanes, d_in, d_out, stream) &
         bind(C, name='dns_four_to_phys_noalloc')

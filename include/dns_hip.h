@@ -214,7 +214,22 @@ int dns_ensemble_destroy(dns_ensemble *e);
 int dns_ensemble_step(dns_ensemble *e, int nsteps);
 /* One RK3 step captured once in a hipGraph on the ensemble's stream and replayed nsteps times. */
 int dns_ensemble_step_graph(dns_ensemble *e, int nsteps);
+/* Waits for the ensemble's stream.  After persistent launches it also reports (DNS_ERR_HIP) a task that
+ * waited too long for the tasks it depends on, in which case those steps' fields are not valid. */
 int dns_ensemble_sync(dns_ensemble *e);
+/* How a substep is launched.  mode 0 (the default): the grouped launches above.  mode 1: one persistent
+ * launch per RK3 substep.  Its workgroups take tasks from a queue in device memory (a task is a run of
+ * blocks of one kernel class for one member; the classes in the order of the grouped launches) and a task
+ * starts once the tasks of its member that it depends on have finished (per-member completion counters),
+ * so different members' classes overlap.  units[c] (c = 0..9: reset, ghost-plane CFL, z-inverse, x-pass u,
+ * x-pass v, CFL reduction, z-forward u, z-forward v, column stage, mean-flow correction; NULL or entries
+ * <= 0: defaults) sets how many units (two plane workgroups, or one column-stage workgroup) a task covers.
+ * The same bodies run, so the results are the same bits as mode 0.  Needs ny - 1 even in every group
+ * (DNS_ERR_ARG otherwise).  Drops a captured step graph.  No counterpart in the reference. */
+int dns_ensemble_set_launch(dns_ensemble *e, int mode, const int *units);
+/* mode in use; with mode 1 also the workgroups of a launch (nwg) and the tasks of substeps 1..3 (ntask[3]);
+ * nwg and ntask may be NULL. */
+int dns_ensemble_get_launch(dns_ensemble *e, int *mode, int *nwg, int *ntask);
 /* Event timing of dns_ensemble_step as dns_set_timing/dns_get_timing: t[0] resets + z-inverse, t[1] x-pass,
  * t[2] z-forward, t[3] column stage (+ CFL reduction, mean-flow correction), t[6] whole substep (ms). */
 int dns_ensemble_set_timing(dns_ensemble *e, int on);

@@ -162,12 +162,14 @@ def ensemble_members():
     return out
 
 
+@pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
-def test_ensemble_48_members_three_sizes(prec):
-    """The 48-member ensemble of three sizes in one grouped launch per kernel class per substep, its RK3 step
-    captured once in a hipGraph and replayed (3 steps): every member equals the same channel run alone
-    (autotuned) bit for bit (own IC, Re, mpgx, CFL dt), and the first member of each size agrees with the
-    oracle (FP64: <= 1e-12; mixed: its FP32 plane stage's error, <= 1e-6 after 3 steps)."""
+def test_ensemble_48_members_three_sizes(prec, launch):
+    """The 48-member ensemble of three sizes, in one grouped launch per kernel class per substep (launch 0)
+    or one persistent launch per substep over a task queue (launch 1), its RK3 step captured once in a
+    hipGraph and replayed (3 steps): every member equals the same channel run alone (autotuned) bit for bit
+    (own IC, Re, mpgx, CFL dt), and the first member of each size agrees with the oracle (FP64: <= 1e-12;
+    mixed: its FP32 plane stage's error, <= 1e-6 after 3 steps)."""
     nsteps = 3
     members = ensemble_members()
     handles, ics = [], []
@@ -183,8 +185,13 @@ def test_ensemble_48_members_three_sizes(prec):
         handles.append(e)
         ics.append(sts)
     ens = dnship.Ensemble(handles)
+    if launch:
+        ens.set_launch(1)
+        mode, nwg, ntask = ens.launch()
+        print("persistent launch: %d workgroups, %s tasks per substep" % (nwg, ntask), flush=True)
+        assert mode == 1 and nwg >= 256 and min(ntask) > 0
     ens.step_graph(nsteps)
-    ens.sync()
+    ens.sync()  # raises if a task of the persistent launch waited too long
     ens.close()
     for (nm, Mx, Mz, ny), grp, sts, e in zip(ENSEMBLE, members, ics, handles):
         print("case: ensemble group %dx%dx%d members vs single channels" % (Mx, Mz, ny), flush=True)
@@ -218,12 +225,14 @@ def test_ensemble_48_members_three_sizes(prec):
         e.close()
 
 
+@pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
-def test_ensemble_grouped_eager_and_graph(prec):
-    """Grouped launches with a constant-flow-rate group (mean-flow correction after the column pass), a
-    rectangular plane and a group of one: 2 eager steps then 2 graph-replayed steps equal every member's
-    single-channel run bit for bit, t, dt, mpgx and iter included; the handles' own downloads wait for the
-    ensemble's work."""
+def test_ensemble_grouped_eager_and_graph(prec, launch):
+    """Grouped launches (launch 0) or the persistent launch (launch 1, one unit per task: the most tasks and
+    waits) with a constant-flow-rate group (mean-flow correction after the column pass), a rectangular
+    plane and a group of one: 2 eager steps then 2 graph-replayed steps equal every member's single-channel
+    run bit for bit, t, dt, mpgx and iter included; the handles' own downloads wait for the ensemble's
+    work."""
     groups = [(48, 48, 33, 0, [(3250.0, -3.0675e-3, 31), (2500.0, -3.5e-3, 32)]),
               (96, 192, 65, 1, [(4000.0, -2.8e-3, 33)]),
               (192, 96, 17, 1, [(3000.0, -3.2e-3, 34), (3500.0, -3.0e-3, 35), (2800.0, -3.3e-3, 36)])]
@@ -238,10 +247,13 @@ def test_ensemble_grouped_eager_and_graph(prec):
             sts.append(st)
         handles.append(h)
         ics.append(sts)
-    print("case: grouped ensemble, 3 groups, prec %d" % prec, flush=True)
+    print("case: ensemble, 3 groups, prec %d, launch %d" % (prec, launch), flush=True)
     ens = dnship.Ensemble(handles)
+    if launch:
+        ens.set_launch(1, units={c: 1 for c in dnship.Ensemble.TASK_CLASSES})
     ens.step(2)
     ens.step_graph(2)
+    ens.sync()
     for (Mx, Mz, ny, ctp, mem), sts, h in zip(groups, ics, handles):
         for m, ((Re, mp, _), st) in enumerate(zip(mem, sts)):
             got, sc = h.download(member=m), h.scalars(member=m)
@@ -270,8 +282,16 @@ def test_ensemble_rejects_unsupported_groups():
     e = dnship.Ensemble([a])
     with pytest.raises(dnship.DnsError):          # no fields uploaded
         e.step(1)
+    with pytest.raises(dnship.DnsError):          # no such launch mode
+        e.set_launch(2)
     e.close()
-    for h in (a, b, c, d):
+    f = dnship.DNS(48, 48, 34)                    # ny - 1 = 33: z-inverse chunks of two planes are not whole
+    e = dnship.Ensemble([a, f])
+    with pytest.raises(dnship.DnsError):
+        e.set_launch(1)
+    assert e.launch()[0] == 0                     # still the grouped launches
+    e.close()
+    for h in (a, b, c, d, f):
         h.close()
 
 
