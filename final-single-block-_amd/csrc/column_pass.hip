@@ -1,8 +1,23 @@
 // One-pass column stage: kernels and launchers of the body in column_pass.h (single channel, and the
 // ensemble's grouped launch).
 #include "column_pass.h"
+#include "column_seq.h"
 
 namespace dnsk {
+
+// The sequential-systems form (column_seq.h): 512 threads per CU (two workgroups of 256).
+template <int NC, int P, int M, typename NT>
+__global__ void __launch_bounds__(NC * P, NC * P >= 512 ? 1 : 512 / (NC * P)) column_seq_kernel(ColArgs a) {
+  using B = ColumnSeqBody<NC, P, M, NT>;
+  __shared__ __align__(16) unsigned char lds[B::LDS];
+  B::run(a, blockIdx.x, gridDim.x, blockIdx.y, threadIdx.x, lds);
+}
+template <int NC, int P, int M>
+static void launch_seq(const ColArgs &a, hipStream_t st) {
+  const dim3 grid((a.C + NC - 1) / NC, a.nm);
+  if (a.Nu1.f32) column_seq_kernel<NC, P, M, float><<<grid, NC * P, 0, st>>>(a);
+  else column_seq_kernel<NC, P, M, double><<<grid, NC * P, 0, st>>>(a);
+}
 
 template <int NC, int P, int M, typename NT>
 __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(ColArgs a) {
@@ -12,9 +27,11 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
 }
 
 // (NC, P, M) by the rows to cover (nyu <= P*M, M >= 3: a segment needs a first, an interior and a last
-// row).  M = 4 keeps a thread within 256 registers (two wavefronts per SIMD); longer segments spill and
-// run one wavefront per SIMD.  Each bucket's shape was timed against alternatives (3-row segments at 96
-// rows, 16 columns x 32 x 5 at the 153-point grids, 8 columns x 64 x 6 at 385 points).
+// row).  With both velocity systems live, M = 4 keeps a thread within 256 registers (two wavefronts per
+// SIMD) and longer segments spill; the sequential form (column_seq.h) holds one system at a time, which
+// lets 257-point grids take 8-row segments in 256-thread workgroups, two per CU.  Each bucket's shape
+// was timed against alternatives (3-row segments at 96 rows, 16 columns x 32 x 5 at the 153-point grids,
+// at 257 points 8 x 32 x 8 sequential against 8 x 64 x 4: 1.69 against 1.80 ms FP64 at 384 x 384).
 __host__ __device__ int column_pass_rows(int nyu) {
   return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 96 ? 96 : nyu <= 128 ? 128 : nyu <= 160 ? 160
          : nyu <= 192 ? 192 : nyu <= 256 ? 256 : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
@@ -31,11 +48,11 @@ static void launch_pass(const ColArgs &a, hipStream_t st) {
 
 // Shapes of a bucket for the autotuner: the same P and M (so the same arithmetic per column, the same
 // bits) with NC = the default, then the other two columns-per-workgroup counts that keep NC * P within
-// 64..512 threads.
+// 64..512 threads (at 385 points: the sequential form and two simultaneous shapes).
 int column_pass_variants(int nyu) {
   switch (column_pass_rows(nyu)) {
-    case 32: case 64: case 96: case 128: case 160: case 384: return 3;
-    case 192: case 256: case 512: return 2;
+    case 32: case 64: case 96: case 128: case 160: case 256: case 384: return 3;
+    case 192: case 512: return 2;
     default: return 0;
   }
 }
@@ -53,9 +70,13 @@ int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
     case 160: if (v == 1) launch_pass<8, 32, 5>(a, st); else if (v == 2) launch_pass<4, 32, 5>(a, st);
               else launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
     case 192: if (v == 1) launch_pass<4, 64, 3>(a, st); else launch_pass<8, 64, 3>(a, st); break;
-    case 256: if (v == 1) launch_pass<4, 64, 4>(a, st); else launch_pass<8, 64, 4>(a, st); break;
-    case 384: if (v == 1) launch_pass<4, 64, 6>(a, st); else if (v == 2) launch_pass<2, 64, 6>(a, st);
-              else launch_pass<8, 64, 6>(a, st); break;
+    // 257-point grids: the three systems one after another, 32 segments of 8 rows (2 workgroups per CU)
+    case 256: if (v == 1) launch_seq<16, 32, 8>(a, st); else if (v == 2) launch_seq<4, 32, 8>(a, st);
+              else launch_seq<8, 32, 8>(a, st); break;
+    // 385-point grids: the sequential form (one system's segment live at a time: 16 spilled registers
+    // instead of 140) and the simultaneous one, same segments, same bits
+    case 384: if (v == 1) launch_pass<8, 64, 6>(a, st); else if (v == 2) launch_pass<4, 64, 6>(a, st);
+              else launch_seq<8, 64, 6>(a, st); break;
     case 512: if (v == 1) launch_pass<2, 64, 8>(a, st); else launch_pass<4, 64, 8>(a, st); break;
     default: return 1;
   }
