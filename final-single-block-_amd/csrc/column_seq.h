@@ -1,8 +1,8 @@
 // One-pass column stage with the three tridiagonal systems eliminated one after another (u1/u3, then
 // u2, then the pressure), for long row segments.  The same substep as ColumnPassBody (column_pass.h) and
 // the same per-row arithmetic (the same reference lines), with a different schedule of the row loop:
-//   phase U : the u-grid rows of the segment (RHS0 + N1, N3 + du for u1, u3), eliminated at once into the
-//             u1/u3 segment; partition solve of that system;
+//   phase U : the u-grid rows of the segment (RHS0 + N1, N3 + du for u1, u3), eliminated into the u1/u3
+//             segment and solved (with 8-row segments one field after the other);
 //   phase V : the v-grid rows (RHS0 + N2 + du for u2), eliminated into the u2 segment; its solve;
 //   phase P : divergence, psi right-hand side, the pressure system; then p += psi, v_corr, wall rows.
 // Only one system's segment state is live at a time, so a thread can hold M = 8 rows (a 256-thread
@@ -29,6 +29,7 @@ struct SeqLds {
 template <int NC, int P, int M, typename NT>
 struct ColumnSeqBody {
   static constexpr int NTH = NC * P;
+  static constexpr bool SPLITU = M >= 8;  // phase U one field at a time (see run)
   static constexpr size_t LDS = sizeof(SeqLds<NC, P, M>);
   __device__ __forceinline__ static void run(const ColArgs &a_, int b, int nblk, int my, int tid,
                                              unsigned char *lds, bool active = true);
@@ -84,10 +85,13 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
   auto cq = [&](int j) { return j < 1 ? 1 : j > nyv ? nyv : j; };
   const double2 z2 = d2(0.0, 0.0);
 
-  // ---- phase U: u-grid rows, u1/u3 segment ---------------------------------------------------------
-  Part<NC, P, M, 2> pu;
-  Segment<M, 2> &su = pu.sg;
-  {
+  // ---- phase U: u-grid rows.  With SPLITU (8-row segments) one field's segment at a time, u1 then u3
+  // (the same matrix: eliminating one right-hand side at a time gives the same bits as both at once, and
+  // keeps an 8-row segment within the registers); otherwise u1 and u3 together ----------------------
+  double2 xu[2][M];  // u1, u3 of the segment's rows after solveU
+  if constexpr (!SPLITU) {
+    Part<NC, P, M, 2> pu;
+    Segment<M, 2> &su2 = pu.sg;
     auto at = [&](int j) { return (size_t)j * C + col; };
     double2 u1m = a.u1[at(cu(j0 - 1))], u1c = a.u1[at(cu(j0))];
     double2 u3m = a.u3[at(cu(j0 - 1))], u3c = a.u3[at(cu(j0))];
@@ -124,7 +128,7 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
       cc = j == nyu ? 0.0 : cc;
       const double2 rr[2] = {ur ? d2(u1c.x + dt * r1.x, u1c.y + dt * r1.y) : z2,
                              ur ? d2(u3c.x + dt * r3.x, u3c.y + dt * r3.y) : z2};
-      su.row(r, ur ? au : 0.0, ur ? bu : 1.0, ur ? cc : 0.0, rr);
+      su2.row(r, ur ? au : 0.0, ur ? bu : 1.0, ur ? cc : 0.0, rr);
       if (live && ur) {
         Nu1.set(oq, N1);
         Nu3.set(oq, N3);
@@ -133,8 +137,79 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
       u3m = u3c; u3c = u3p;
       uvm = uvc; wvm = wvc;
     }
+    seq_solve<NC, P, M, 2>(pu, U2, c, s);
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      xu[0][r] = su2.d[0][r];
+      xu[1][r] = su2.d[1][r];
+    }
+  } else {
+    // field F = 0 (u1: flux uv, Nu1, Nxz1, the kx pressure term and the mean pressure gradient) or
+    // F = 1 (u3: flux wv, Nu3, Nxz3, the kz pressure term); the same operations per field as above
+    auto uphase = [&](auto fi) {
+      constexpr int F = decltype(fi)::value;
+      Part<NC, P, M, 1> pf;
+      Segment<M, 1> &sf = pf.sg;
+      const double2 *uf = F == 0 ? a.u1 : a.u3;
+      const NA<NT> &fl = F == 0 ? uvf : wvf;
+      const NA<NT> &nuo = F == 0 ? Nu1 : Nu3;
+      const NA<NT> &nxz = F == 0 ? Nxz1 : Nxz3;
+      const double c3 = F == 0 ? c3x : c3z;
+      const int jb = F == 0 ? j0 : opaque(j0), cl = F == 0 ? col : opaque(col);
+      auto at = [&](int j) { return (size_t)j * C + cl; };
+      double2 um = uf[at(cu(jb - 1))], uc = uf[at(cu(jb))];
+      double2 fm = fl[at(cv(jb - 1))];
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        const int j = jb + r;
+        const bool ur = j <= nyu;
+        const int ju = cp(j);
+        double2 pc;
+        if constexpr (F == 0) {
+          pc = a.p[at(ju - 1)];
+          pst[r * NTH + tid] = pc;  // p row cp(j)
+        } else {
+          pc = pst[r * NTH + tid];
+        }
+        const double2 up = uf[at(cu(j + 1))];
+        const double2 fc = fl[at(cv(j))];
+        const size_t oq = at(ju - 1);
+        const double2 nu = nuo[oq], nx = nxz[oq];
+        const double g1 = D2U(1, ju), g2 = D2U(2, ju), g3 = D2U(3, ju), hu = a.dthdyu[ju];
+        const double dd2 = k2 + g2;
+        const double2 Lf = cadd(cadd(rmul(g1, um), rmul(dd2, uc)), rmul(g3, up));
+        double2 rf = cadd(cadd(rmul(C1, Lf), rmul(C2, nu)), imul(c3, pc));
+        if constexpr (F == 0) rf.x = rf.x + Cm;
+        const double2 Nf = cadd(nx, rmul(hu, csub(fc, fm)));
+        rf = cadd(rf, rmul(Cg, Nf));
+        double au = -beta * g1, bu = 1.0 - beta * (g2 + m.k2x + m.k2z), cc = -beta * g3;
+        const double bf = bu - au * a.gw1;
+        bu = j == 1 ? bf : bu;
+        au = j == 1 ? 0.0 : au;
+        const double bl = bu - cc * a.gw2;
+        bu = j == nyu ? bl : bu;
+        cc = j == nyu ? 0.0 : cc;
+        const double2 rr = ur ? d2(uc.x + dt * rf.x, uc.y + dt * rf.y) : z2;
+        sf.row(r, ur ? au : 0.0, ur ? bu : 1.0, ur ? cc : 0.0, &rr);
+        if (live && ur) nuo.set(oq, Nf);
+        um = uc; uc = up;
+        fm = fc;
+      }
+      seq_solve<NC, P, M, 1>(pf, U1, c, s);
+#pragma unroll
+      for (int r = 0; r < M; r++) xu[F][r] = sf.d[0][r];
+    };
+    uphase(std::integral_constant<int, 0>{});
+    uphase(std::integral_constant<int, 1>{});
   }
-  seq_solve<NC, P, M, 2>(pu, U2, c, s);
+  struct {
+    double2 d[2][M];
+  } su;
+#pragma unroll
+  for (int r = 0; r < M; r++) {
+    su.d[0][r] = xu[0][r];
+    su.d[1][r] = xu[1][r];
+  }
   // su.d[0][r], su.d[1][r]: u1, u3 of row j0+r after solveU
 
   // ---- phase V: v-grid rows, u2 segment ------------------------------------------------------------
