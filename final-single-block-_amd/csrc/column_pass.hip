@@ -30,8 +30,9 @@ __global__ void __launch_bounds__(NC * P, M <= 4 ? 2 : 1) column_pass_kernel(Col
 // row).  With both velocity systems live, M = 4 keeps a thread within 256 registers (two wavefronts per
 // SIMD) and longer segments spill; the sequential form (column_seq.h) holds one system at a time, which
 // lets 257-point grids take 8-row segments in 256-thread workgroups, two per CU.  Each bucket's shape
-// was timed against alternatives (3-row segments at 96 rows, 16 columns x 32 x 5 at the 153-point grids,
-// at 257 points 8 x 32 x 8 sequential against 8 x 64 x 4: 1.69 against 1.80 ms FP64 at 384 x 384).
+// was timed against alternatives (3-row segments at 96 rows; at the 153-point grids 32 x 5 rows, in the
+// sequential form 0.135 ms against 0.170 for 16 simultaneous columns at 128 x 128; at 257 points 8 x 32 x 8
+// sequential against 8 x 64 x 4: 1.61 against 1.80 ms FP64 at 384 x 384).
 __host__ __device__ int column_pass_rows(int nyu) {
   return nyu <= 32 ? 32 : nyu <= 64 ? 64 : nyu <= 96 ? 96 : nyu <= 128 ? 128 : nyu <= 160 ? 160
          : nyu <= 192 ? 192 : nyu <= 256 ? 256 : nyu <= 384 ? 384 : nyu <= 512 ? 512 : 0;
@@ -67,8 +68,8 @@ int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
              else launch_pass<8, 32, 3>(a, st); break;
     case 128: if (v == 1) launch_pass<16, 32, 4>(a, st); else if (v == 2) launch_pass<4, 32, 4>(a, st);
               else launch_pass<8, 32, 4>(a, st); break;
-    case 160: if (v == 1) launch_pass<8, 32, 5>(a, st); else if (v == 2) launch_pass<4, 32, 5>(a, st);
-              else launch_pass<16, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
+    case 160: if (v == 1) launch_pass<16, 32, 5>(a, st); else if (v == 2) launch_seq<16, 32, 5>(a, st);
+              else launch_seq<8, 32, 5>(a, st); break;  // 153-point grids: 32 x 5 rows instead of 64 x 4
     case 192: if (v == 1) launch_pass<4, 64, 3>(a, st); else launch_pass<8, 64, 3>(a, st); break;
     // 257-point grids: the three systems one after another, 32 segments of 8 rows (2 workgroups per CU)
     case 256: if (v == 1) launch_seq<16, 32, 8>(a, st); else if (v == 2) launch_seq<4, 32, 8>(a, st);
@@ -84,22 +85,25 @@ int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
 }
 
 // ---- grouped launch of the ensemble: every member of every size group in one launch -----------------
-// A 512-thread workgroup runs 512/(NC*P) bodies of the group's single-channel shape side by side (each with
+// A 512-thread workgroup runs 512/NTH bodies of the group's single-channel shape side by side (each with
 // its own slice of LDS and its own column block), so every member gets exactly the arithmetic of the
-// single-channel launch; the shapes of rows <= 160 take 64 to 512 threads and about 116 KB of LDS per
-// 512 threads.  Row buckets as column_pass_rows: 32 (8 x 8 x 4), 64 (8 x 16 x 4), 96 (8 x 32 x 3),
-// 128 (8 x 32 x 4), 160 (16 x 32 x 5).
+// single-channel launch.  Row buckets as column_pass_rows: 32 (8 x 8 x 4), 64 (8 x 16 x 4), 96
+// (8 x 32 x 3), 128 (8 x 32 x 4), 160 (the sequential form, 8 x 32 x 5: two bodies).
 constexpr int ENS_COL_NTH = 512;
-template <int NC, int P, int M, typename NT>
+template <class B>
 struct EnsColShape {
-  using B = ColumnPassBody<NC, P, M, NT>;
   static constexpr int SUB = ENS_COL_NTH / B::NTH;   // bodies per workgroup
   static constexpr size_t LDS = SUB * ((B::LDS + 15) / 16 * 16);
 };
+template <typename NT> using EnsCol32 = ColumnPassBody<8, 8, 4, NT>;
+template <typename NT> using EnsCol64 = ColumnPassBody<8, 16, 4, NT>;
+template <typename NT> using EnsCol96 = ColumnPassBody<8, 32, 3, NT>;
+template <typename NT> using EnsCol128 = ColumnPassBody<8, 32, 4, NT>;
+template <typename NT> using EnsCol160 = ColumnSeqBody<8, 32, 5, NT>;
 template <typename NT> constexpr size_t ens_col_lds() {
-  constexpr size_t l[5] = {EnsColShape<8, 8, 4, NT>::LDS, EnsColShape<8, 16, 4, NT>::LDS,
-                           EnsColShape<8, 32, 3, NT>::LDS, EnsColShape<8, 32, 4, NT>::LDS,
-                           EnsColShape<16, 32, 5, NT>::LDS};
+  constexpr size_t l[5] = {EnsColShape<EnsCol32<NT>>::LDS, EnsColShape<EnsCol64<NT>>::LDS,
+                           EnsColShape<EnsCol96<NT>>::LDS, EnsColShape<EnsCol128<NT>>::LDS,
+                           EnsColShape<EnsCol160<NT>>::LDS};
   size_t m = 0;
   for (size_t v : l) m = v > m ? v : m;
   return m;
@@ -107,17 +111,17 @@ template <typename NT> constexpr size_t ens_col_lds() {
 
 bool ens_column_rows_supported(int nyu) { return nyu >= 8 && nyu <= 160; }
 
-template <int NC, int P, int M, typename NT>
+template <class B>
 __device__ __forceinline__ void ens_col_run(const ColArgs &a, int lb, int nb, unsigned char *lds) {
-  using S = EnsColShape<NC, P, M, NT>;
-  const int sub = threadIdx.x / S::B::NTH, tid = threadIdx.x % S::B::NTH;
+  using S = EnsColShape<B>;
+  const int sub = threadIdx.x / B::NTH, tid = threadIdx.x % B::NTH;
   // workgroup lb holds column blocks lb*SUB .. lb*SUB + SUB-1 of the group (member-major); a body past the
   // group's last block computes the last block again (every barrier is reached) and stores nothing
   const int gb = lb * S::SUB + sub;
   const int total = nb * a.nm;
   const bool active = gb < total;
   const int b = active ? gb : total - 1;
-  S::B::run(a, b % nb, nb, b / nb, tid, lds + sub * ((S::B::LDS + 15) / 16 * 16), active);
+  B::run(a, b % nb, nb, b / nb, tid, lds + sub * ((B::LDS + 15) / 16 * 16), active);
 }
 
 template <typename NT>
@@ -128,11 +132,11 @@ __global__ void __launch_bounds__(ENS_COL_NTH, 1) ens_column_pass_kernel(EnsColL
   while (g + 1 < E.ng && b >= E.off[g + 1]) g++;
   const int lb = b - E.off[g], nb = E.nblk[g];
   switch (column_pass_rows(E.a[g].nyu)) {
-    case 32: ens_col_run<8, 8, 4, NT>(E.a[g], lb, nb, lds); break;
-    case 64: ens_col_run<8, 16, 4, NT>(E.a[g], lb, nb, lds); break;
-    case 96: ens_col_run<8, 32, 3, NT>(E.a[g], lb, nb, lds); break;
-    case 128: ens_col_run<8, 32, 4, NT>(E.a[g], lb, nb, lds); break;
-    default: ens_col_run<16, 32, 5, NT>(E.a[g], lb, nb, lds); break;
+    case 32: ens_col_run<EnsCol32<NT>>(E.a[g], lb, nb, lds); break;
+    case 64: ens_col_run<EnsCol64<NT>>(E.a[g], lb, nb, lds); break;
+    case 96: ens_col_run<EnsCol96<NT>>(E.a[g], lb, nb, lds); break;
+    case 128: ens_col_run<EnsCol128<NT>>(E.a[g], lb, nb, lds); break;
+    default: ens_col_run<EnsCol160<NT>>(E.a[g], lb, nb, lds); break;
   }
 }
 
@@ -142,8 +146,8 @@ int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st) {
     const int nyu = E.a[g].nyu;
     if (!ens_column_rows_supported(nyu)) return -1;
     const int rows = column_pass_rows(nyu);
-    const int nc = rows == 160 ? 16 : 8;
-    const int sub = rows == 32 ? 8 : rows == 64 ? 4 : rows == 160 ? 1 : 2;   // bodies per workgroup
+    const int nc = 8;
+    const int sub = rows == 32 ? 8 : rows == 64 ? 4 : 2;   // bodies per workgroup
     E.nblk[g] = (E.a[g].C + nc - 1) / nc;                                  // column blocks per member
     E.off[g] = total;
     total += (E.nblk[g] * E.a[g].nm + sub - 1) / sub;

@@ -236,7 +236,7 @@ struct EnsSmall {
   int nm[ENS_MAXG], C[ENS_MAXG], gx[ENS_MAXG], off[ENS_MAXG + 1];
   int ng;
 };
-// one grouped column-pass launch (shape by row count: rows <= 96 or <= 160, 512 threads)
+// one grouped column-pass launch (shape by row count, up to 160 rows; 512-thread workgroups)
 struct EnsColLaunch {
   ColArgs a[ENS_MAXG];
   int nblk[ENS_MAXG], off[ENS_MAXG + 1];

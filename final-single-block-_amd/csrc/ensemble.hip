@@ -10,6 +10,7 @@
 // tile variants of a kernel compute the same bits, so a member's result equals the same channel run alone.
 #include "plane_launch.h"
 #include "column_pass.h"
+#include "column_seq.h"
 
 namespace dnsk {
 
@@ -252,7 +253,9 @@ template <> struct PCol<96> { static constexpr int P = 32, M = 3; };
 template <> struct PCol<128> { static constexpr int P = 32, M = 4; };
 template <> struct PCol<160> { static constexpr int P = 32, M = 5; };
 template <int ROWS, typename NT> struct PColB {
-  using B = ColumnPassBody<8, PCol<ROWS>::P, PCol<ROWS>::M, NT>;
+  // the sequential form at 160 rows (as the single-channel and grouped launches: the same bits)
+  using B = typename std::conditional<ROWS == 160, ColumnSeqBody<8, PCol<ROWS>::P, PCol<ROWS>::M, NT>,
+                                      ColumnPassBody<8, PCol<ROWS>::P, PCol<ROWS>::M, NT>>::type;
   static constexpr int SUB = PERSIST_NTH / B::NTH;
   static constexpr size_t LDS = SUB * al16(B::LDS);
 };
