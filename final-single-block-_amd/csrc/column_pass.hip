@@ -85,11 +85,12 @@ int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
 }
 
 // ---- grouped launch of the ensemble: every member of every size group in one launch -----------------
-// A 512-thread workgroup runs 512/NTH bodies of the group's single-channel shape side by side (each with
-// its own slice of LDS and its own column block), so every member gets exactly the arithmetic of the
-// single-channel launch.  Row buckets as column_pass_rows: 32 (8 x 8 x 4), 64 (8 x 16 x 4), 96
-// (8 x 32 x 3), 128 (8 x 32 x 4), 160 (the sequential form, 8 x 32 x 5: two bodies).
-constexpr int ENS_COL_NTH = 512;
+// A 256-thread workgroup (two per CU) runs 256/NTH bodies of the group's single-channel shape side by side
+// (each with its own slice of LDS and its own column block), so every member gets exactly the arithmetic
+// of the single-channel launch.  Row buckets as column_pass_rows: 32 (8 x 8 x 4: four bodies), 64
+// (8 x 16 x 4: two), 96 (8 x 32 x 3), 128 (8 x 32 x 4), 160 (the sequential form, 8 x 32 x 5).  512-thread
+// workgroups (one per CU) took 1.32 against 1.24 ms per substep at the 48-member configuration (FP64).
+constexpr int ENS_COL_NTH = 256;
 template <class B>
 struct EnsColShape {
   static constexpr int SUB = ENS_COL_NTH / B::NTH;   // bodies per workgroup
@@ -125,7 +126,7 @@ __device__ __forceinline__ void ens_col_run(const ColArgs &a, int lb, int nb, un
 }
 
 template <typename NT>
-__global__ void __launch_bounds__(ENS_COL_NTH, 1) ens_column_pass_kernel(EnsColLaunch E) {
+__global__ void __launch_bounds__(ENS_COL_NTH, 2) ens_column_pass_kernel(EnsColLaunch E) {
   __shared__ __align__(16) unsigned char lds[ens_col_lds<NT>()];
   int g = 0;
   const int b = blockIdx.x;
@@ -147,7 +148,7 @@ int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st) {
     if (!ens_column_rows_supported(nyu)) return -1;
     const int rows = column_pass_rows(nyu);
     const int nc = 8;
-    const int sub = rows == 32 ? 8 : rows == 64 ? 4 : 2;   // bodies per workgroup
+    const int sub = rows == 32 ? 4 : rows == 64 ? 2 : 1;   // bodies per workgroup
     E.nblk[g] = (E.a[g].C + nc - 1) / nc;                                  // column blocks per member
     E.off[g] = total;
     total += (E.nblk[g] * E.a[g].nm + sub - 1) / sub;
