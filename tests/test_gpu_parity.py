@@ -97,6 +97,9 @@ def test_fused_steps_vs_reference_golden(golden, ctp, path):
     (24, 24, 385, 2, 8),      # 385 points (one-pass column stage with 6-row segments)
     (24, 24, 193, 2, 10),     # 64 segments of 3 rows
     (24, 24, 513, 2, 11),     # 64 segments of 8 rows
+    (24, 24, 131, 2, 16),     # 160-row bucket (sequential form, 32 x 5) with 30 identity rows at the top
+    (24, 24, 201, 2, 17),     # 256-row bucket (sequential form, 32 x 8): the last 7 segments past nyu
+    (24, 24, 301, 2, 18),     # 384-row bucket (sequential form, 64 x 6): 84 identity rows
     (12, 12, 17, 3, 12),      # the smallest plane, 8 x 8 modes
     (144, 288, 9, 1, 13),     # Mx != Mz: x half-length 72 (plan 9.8), z 288
     (288, 144, 9, 1, 14),     # x half-length 144, z 144
