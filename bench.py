@@ -101,6 +101,7 @@ def ensemble_leg(steps, warmup, prec, eager=False):
                 g.upload(*st, member=m)
 
     ens = dnship.Ensemble(handles)
+    tvar, twg, tms = ens.autotune(iters=5)   # tile shapes of the grouped launches, timed on this workload
     ens_step = ens.step if eager else ens.step_graph
     ens_step(max(warmup, 1))
     ens.sync()
@@ -143,8 +144,10 @@ def ensemble_leg(steps, warmup, prec, eager=False):
         g.close()
     return {"value": 1.0 / sec, "unit": "ensemble steps/s (all 48 members advance one RK3 step)",
             "ms_per_step": sec * 1e3, "members": idx, "groups": [list(x) for x in ENSEMBLE],
-            "launch": "one grouped launch per kernel class per substep for all members, "
-                      + ("eager steps" if eager else "step replayed from a hipGraph"),
+            "launch": "one grouped launch per kernel class per substep for all members, tile shapes from the "
+                      "ensemble autotuner, " + ("eager steps" if eager else "step replayed from a hipGraph"),
+            "tiles": {"classes": list(dnship.Ensemble.TILE_CLASSES), "variant": tvar, "workgroup": twg,
+                      "ms_per_shape": [[round(t, 4) for t in c] for c in tms]},
             "kernels_ms_per_substep": {k: v for k, v in tim.items() if k != "n"},
             "algorithmic_GB_per_step": by / 1e9, "frac_hbm": by / sec / 1e9 / dns_roofline.HBM_PEAK_GBS,
             "persistent": {"value": 1.0 / sec_persist, "ms_per_step": sec_persist * 1e3,

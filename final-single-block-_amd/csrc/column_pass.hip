@@ -85,36 +85,60 @@ int launch_column_pass(const ColArgs &a, int v, hipStream_t st) {
 }
 
 // ---- grouped launch of the ensemble: every member of every size group in one launch -----------------
-// A 256-thread workgroup (two per CU) runs 256/NTH bodies of the group's single-channel shape side by side
-// (each with its own slice of LDS and its own column block), so every member gets exactly the arithmetic
-// of the single-channel launch.  Row buckets as column_pass_rows: 32 (8 x 8 x 4: four bodies), 64
-// (8 x 16 x 4: two), 96 (8 x 32 x 3), 128 (8 x 32 x 4), 160 (the sequential form, 8 x 32 x 5).  512-thread
-// workgroups (one per CU) took 1.32 against 1.24 ms per substep at the 48-member configuration (FP64).
+// A 256-thread workgroup (two per CU) runs 256/NTH bodies of the group's shape side by side (each with
+// its own slice of LDS and its own column block), so every member gets exactly the arithmetic of the
+// single-channel launch.  Shapes by row bucket (8 <= nyu <= 160), three variants with the same segments
+// (P, M) in each bucket, so the same bits; a launch takes one variant for all groups (each variant its
+// own kernel, with the registers of its own bodies), chosen by the ensemble autotuner (0 by default):
+//    32 rows:  8, 16 or 32 columns x 8 segments x 4 rows        (64, 128, 256 threads)
+//    64 rows:  8, 16 or 4 columns x 16 x 4
+//    96 rows:  8, 4 or 2 columns x 32 x 3
+//   128 rows:  8 or 4 columns x 32 x 4, or 8 x 32 x 4 in the sequential form
+//   160 rows:  8 or 4 columns x 32 x 5 in the sequential form, or 8 x 32 x 5 simultaneous
+// 512-thread workgroups (one per CU) took 1.32 against 1.24 ms per substep at the 48-member configuration
+// (FP64).
 constexpr int ENS_COL_NTH = 256;
-template <class B>
-struct EnsColShape {
+template <int ROWS, int V, typename NT> struct EnsColB;
+template <int V, typename NT> struct EnsColB<32, V, NT> { using B = ColumnPassBody<8 << V, 8, 4, NT>; };
+template <int V, typename NT> struct EnsColB<64, V, NT> {
+  using B = ColumnPassBody<V == 0 ? 8 : V == 1 ? 16 : 4, 16, 4, NT>;
+};
+template <int V, typename NT> struct EnsColB<96, V, NT> { using B = ColumnPassBody<(8 >> V), 32, 3, NT>; };
+template <int V, typename NT> struct EnsColB<128, V, NT> {
+  using B = typename std::conditional<V == 2, ColumnSeqBody<8, 32, 4, NT>, ColumnPassBody<(8 >> V), 32, 4, NT>>::type;
+};
+template <int V, typename NT> struct EnsColB<160, V, NT> {
+  using B = typename std::conditional<V == 2, ColumnPassBody<8, 32, 5, NT>, ColumnSeqBody<(8 >> V), 32, 5, NT>>::type;
+};
+template <int ROWS, int V, typename NT> struct EnsColShape {
+  using B = typename EnsColB<ROWS, V, NT>::B;
+  static_assert(ENS_COL_NTH % B::NTH == 0, "whole bodies per workgroup");
   static constexpr int SUB = ENS_COL_NTH / B::NTH;   // bodies per workgroup
   static constexpr size_t LDS = SUB * ((B::LDS + 15) / 16 * 16);
 };
-template <typename NT> using EnsCol32 = ColumnPassBody<8, 8, 4, NT>;
-template <typename NT> using EnsCol64 = ColumnPassBody<8, 16, 4, NT>;
-template <typename NT> using EnsCol96 = ColumnPassBody<8, 32, 3, NT>;
-template <typename NT> using EnsCol128 = ColumnPassBody<8, 32, 4, NT>;
-template <typename NT> using EnsCol160 = ColumnSeqBody<8, 32, 5, NT>;
-template <typename NT> constexpr size_t ens_col_lds() {
-  constexpr size_t l[5] = {EnsColShape<EnsCol32<NT>>::LDS, EnsColShape<EnsCol64<NT>>::LDS,
-                           EnsColShape<EnsCol96<NT>>::LDS, EnsColShape<EnsCol128<NT>>::LDS,
-                           EnsColShape<EnsCol160<NT>>::LDS};
+template <int V, typename NT> constexpr size_t ens_col_lds() {
   size_t m = 0;
-  for (size_t v : l) m = v > m ? v : m;
+  for (size_t v : {EnsColShape<32, V, NT>::LDS, EnsColShape<64, V, NT>::LDS, EnsColShape<96, V, NT>::LDS,
+                   EnsColShape<128, V, NT>::LDS, EnsColShape<160, V, NT>::LDS})
+    m = v > m ? v : m;
   return m;
 }
+static int ens_col_nc(int rows, int v) {  // columns per body
+  switch (rows) {
+    case 32: return 8 << v;
+    case 64: return v == 0 ? 8 : v == 1 ? 16 : 4;
+    case 96: return 8 >> v;
+    default: return v == 1 ? 4 : 8;
+  }
+}
+static int ens_col_nth(int rows, int v) { return ens_col_nc(rows, v) * (rows == 32 ? 8 : rows == 64 ? 16 : 32); }
 
 bool ens_column_rows_supported(int nyu) { return nyu >= 8 && nyu <= 160; }
 
-template <class B>
+template <int ROWS, int V, typename NT>
 __device__ __forceinline__ void ens_col_run(const ColArgs &a, int lb, int nb, unsigned char *lds) {
-  using S = EnsColShape<B>;
+  using S = EnsColShape<ROWS, V, NT>;
+  using B = typename S::B;
   const int sub = threadIdx.x / B::NTH, tid = threadIdx.x % B::NTH;
   // workgroup lb holds column blocks lb*SUB .. lb*SUB + SUB-1 of the group (member-major); a body past the
   // group's last block computes the last block again (every barrier is reached) and stores nothing
@@ -125,37 +149,43 @@ __device__ __forceinline__ void ens_col_run(const ColArgs &a, int lb, int nb, un
   B::run(a, b % nb, nb, b / nb, tid, lds + sub * ((B::LDS + 15) / 16 * 16), active);
 }
 
-template <typename NT>
+template <int V, typename NT>
 __global__ void __launch_bounds__(ENS_COL_NTH, 2) ens_column_pass_kernel(EnsColLaunch E) {
-  __shared__ __align__(16) unsigned char lds[ens_col_lds<NT>()];
+  __shared__ __align__(16) unsigned char lds[ens_col_lds<V, NT>()];
   int g = 0;
   const int b = blockIdx.x;
   while (g + 1 < E.ng && b >= E.off[g + 1]) g++;
   const int lb = b - E.off[g], nb = E.nblk[g];
   switch (column_pass_rows(E.a[g].nyu)) {
-    case 32: ens_col_run<EnsCol32<NT>>(E.a[g], lb, nb, lds); break;
-    case 64: ens_col_run<EnsCol64<NT>>(E.a[g], lb, nb, lds); break;
-    case 96: ens_col_run<EnsCol96<NT>>(E.a[g], lb, nb, lds); break;
-    case 128: ens_col_run<EnsCol128<NT>>(E.a[g], lb, nb, lds); break;
-    default: ens_col_run<EnsCol160<NT>>(E.a[g], lb, nb, lds); break;
+    case 32: ens_col_run<32, V, NT>(E.a[g], lb, nb, lds); break;
+    case 64: ens_col_run<64, V, NT>(E.a[g], lb, nb, lds); break;
+    case 96: ens_col_run<96, V, NT>(E.a[g], lb, nb, lds); break;
+    case 128: ens_col_run<128, V, NT>(E.a[g], lb, nb, lds); break;
+    default: ens_col_run<160, V, NT>(E.a[g], lb, nb, lds); break;
   }
 }
 
-int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st) {
+int launch_ens_column_pass(EnsColLaunch E, bool fp32, int v, hipStream_t st) {
+  if (v < 0 || v >= ENS_NVAR) return -1;
   int total = 0;
   for (int g = 0; g < E.ng; g++) {
     const int nyu = E.a[g].nyu;
     if (!ens_column_rows_supported(nyu)) return -1;
     const int rows = column_pass_rows(nyu);
-    const int nc = 8;
-    const int sub = rows == 32 ? 4 : rows == 64 ? 2 : 1;   // bodies per workgroup
+    const int nc = ens_col_nc(rows, v);
+    const int sub = ENS_COL_NTH / ens_col_nth(rows, v);                   // bodies per workgroup
     E.nblk[g] = (E.a[g].C + nc - 1) / nc;                                  // column blocks per member
     E.off[g] = total;
     total += (E.nblk[g] * E.a[g].nm + sub - 1) / sub;
   }
   E.off[E.ng] = total;
-  if (fp32) ens_column_pass_kernel<float><<<total, ENS_COL_NTH, 0, st>>>(E);
-  else ens_column_pass_kernel<double><<<total, ENS_COL_NTH, 0, st>>>(E);
+#define GO(V)                                                           \
+  if (fp32) ens_column_pass_kernel<V, float><<<total, ENS_COL_NTH, 0, st>>>(E); \
+  else ens_column_pass_kernel<V, double><<<total, ENS_COL_NTH, 0, st>>>(E);
+  if (v == 0) { GO(0) }
+  else if (v == 1) { GO(1) }
+  else { GO(2) }
+#undef GO
   return 0;
 }
 

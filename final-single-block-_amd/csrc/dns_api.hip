@@ -840,6 +840,11 @@ struct dns_ensemble {
   unsigned *pctr = nullptr;            // device: queue head, finished workgroups, timeout flag, counters
   int nwg = 0;                         // workgroups of a persistent launch
   int ntask[3] = {};
+  // tile shapes of the grouped launches (dns_ensemble_set_tiles, dns_ensemble_autotune): variant of the
+  // plane classes c = 0 z-inverse, 1/2 x-pass, 3/4 z-forward and of the column pass (var[5]); threads per
+  // workgroup of the plane classes
+  int var[6] = {1, 1, 1, 1, 1, 0};
+  int wg[5] = {128, 128, 128, 128, 128};
 };
 
 static void ens_mark(dns_ensemble *e, int q) {
@@ -878,19 +883,20 @@ static int ens_substep(dns_ensemble *e, int kRK) {
     HIPCHK(hipGetLastError());
     return DNS_OK;
   }
+  auto plane = [&](int c) { return dnsk::launch_ens_plane(P, e->fp32, c, e->wg[c], e->var[c], e->stream); };
   ens_mark(e, 0);
   dnsk::launch_ens_reset(S, kRK, e->stream);
   if (kRK == 1) dnsk::launch_ens_ghost(S, e->stream);
-  int rc = dnsk::launch_ens_plane(P, e->fp32, 0, e->stream);
+  int rc = plane(0);
   ens_mark(e, 1);
-  rc |= dnsk::launch_ens_plane(P, e->fp32, 1, e->stream);
-  rc |= dnsk::launch_ens_plane(P, e->fp32, 2, e->stream);
+  rc |= plane(1);
+  rc |= plane(2);
   ens_mark(e, 2);
-  rc |= dnsk::launch_ens_plane(P, e->fp32, 3, e->stream);
-  rc |= dnsk::launch_ens_plane(P, e->fp32, 4, e->stream);
+  rc |= plane(3);
+  rc |= plane(4);
   ens_mark(e, 3);
   if (kRK == 1) dnsk::launch_ens_cfl_reduce(S, e->stream);
-  rc |= dnsk::launch_ens_column_pass(Cl, e->fp32, e->stream);
+  rc |= dnsk::launch_ens_column_pass(Cl, e->fp32, e->var[5], e->stream);
   dnsk::launch_ens_meanflow_post(Cl, flags, e->stream);
   ens_mark(e, 4);
   if (rc) return fail(DNS_ERR_ARG, "ensemble: a size is not instantiated for grouped launches");
@@ -1186,6 +1192,54 @@ extern "C" int dns_set_plane_variants(dns_handle *h, const int *var) {
   return DNS_OK;
 }
 
+// Copies of what the column sweeps overwrite (u1, u3, u2, Nu1..3, p of every member, and the members'
+// scalars), kept by the autotuners while they time and put back afterwards, on stream st.
+struct TuneSave {
+  double2 *buf = nullptr;
+  std::vector<DnsState> st;
+};
+static void tune_arrays(dns_handle *h, double2 **src, size_t *len) {
+  const size_t C = h->C;
+  const size_t nu = C * (h->nyu + 2) * h->nm, nv = C * (h->nyv + 2) * h->nm, nn = C * h->nyu * h->nm;
+  double2 *a[7] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3, h->p};
+  const size_t l[7] = {nu, nu, nv, nn, nn, nn, nn};
+  for (int k = 0; k < 7; k++) {
+    src[k] = a[k];
+    len[k] = l[k];
+  }
+}
+static int tune_save(dns_handle *h, hipStream_t st, TuneSave &sv) {
+  double2 *src[7];
+  size_t len[7], tot = 0;
+  tune_arrays(h, src, len);
+  for (size_t l : len) tot += l;
+  sv.st.resize(h->nm);
+  HIPCHK(hipMalloc(&sv.buf, tot * sizeof(double2)));
+  HIPCHK(hipMemcpyAsync(sv.st.data(), h->st, h->nm * sizeof(DnsState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double2 *q = sv.buf;
+  for (int k = 0; k < 7; k++) {
+    HIPCHK(hipMemcpyAsync(q, src[k], len[k] * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    q += len[k];
+  }
+  return DNS_OK;
+}
+static void tune_restore(dns_handle *h, hipStream_t st, TuneSave &sv) {
+  if (!sv.buf) return;
+  double2 *src[7];
+  size_t len[7];
+  tune_arrays(h, src, len);
+  double2 *q = sv.buf;
+  for (int k = 0; k < 7; k++) {
+    (void)hipMemcpyAsync(src[k], q, len[k] * sizeof(double2), hipMemcpyDeviceToDevice, st);
+    q += len[k];
+  }
+  (void)hipMemcpyAsync(h->st, sv.st.data(), h->nm * sizeof(DnsState), hipMemcpyHostToDevice, st);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(sv.buf);
+  sv.buf = nullptr;
+}
+
 // Times every variant of every kernel that has more than one and keeps the fastest: plane kernels on
 // the first plane batch with kRK = 2 arguments (no CFL maxima), the column sweep on the whole field.
 // Rewrites scratch only (nonlinear terms, fluxes, eliminated rows), so it may be called before a step
@@ -1202,23 +1256,8 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   // the column sweeps overwrite u1, u2, u3, p and Nu: keep copies and put them back afterwards
-  const size_t C = h->C;
-  const size_t nu = C * (h->nyu + 2) * h->nm, nv = C * (h->nyv + 2) * h->nm, nn = C * h->nyu * h->nm;
-  double2 *save = nullptr;
-  int rc = DNS_OK;
-  const double2 *src[7] = {h->u1, h->u3, h->u2, h->Nu1, h->Nu2, h->Nu3, h->p};
-  const size_t len[7] = {nu, nu, nv, nn, nn, nn, nn};
-  std::vector<DnsState> st_save(h->nm);
-  {
-    HIPCHK(hipMalloc(&save, (2 * nu + nv + 4 * nn) * sizeof(double2)));
-    HIPCHK(hipMemcpyAsync(st_save.data(), h->st, h->nm * sizeof(DnsState), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    double2 *q = save;
-    for (int k = 0; k < 7; k++) {
-      HIPCHK(hipMemcpyAsync(q, src[k], len[k] * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-      q += len[k];
-    }
-  }
+  TuneSave sv;
+  int rc = tune_save(h, h->stream, sv);
   ColArgs ca = col_args(h, 2);
   for (int c = 0; c < DNS_NKERN && rc == DNS_OK; c++) {
     const int nw = c == 0 ? a.nbj + 1 : a.nbj;  // the first plane batch
@@ -1254,22 +1293,128 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
     h->var[c] = best;
     if (chosen) chosen[c] = best;
   }
-  if (save) {
-    double2 *q = save;
-    for (int k = 0; k < 7; k++) {
-      (void)hipMemcpyAsync(const_cast<double2 *>(src[k]), q, len[k] * sizeof(double2), hipMemcpyDeviceToDevice,
-                           h->stream);
-      q += len[k];
-    }
-    (void)hipMemcpyAsync(h->st, st_save.data(), h->nm * sizeof(DnsState), hipMemcpyHostToDevice, h->stream);
-    (void)hipStreamSynchronize(h->stream);
-    (void)hipFree(save);
-  }
+  tune_restore(h, h->stream, sv);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   drop_graph(h);
   if (rc == DNS_OK) HIPCHK(hipGetLastError());
   return rc;
+}
+
+// ---- tiles of the ensemble's grouped launches ------------------------------------------------------
+static void ens_drop_graph(dns_ensemble *e) {
+  if (e->graph) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipGraphExecDestroy(e->graph);
+  }
+  e->graph = nullptr;
+}
+static PlaneArgs ens_plane_args(dns_handle *h, int kRK) {
+  PlaneArgs a = plane_args(h, kRK);
+  a.j0 = 1;
+  a.nbj = h->nyu;
+  return a;
+}
+
+// shape k = 0..4 of a plane class: (workgroup size, variant)
+static const int ENS_SHAPE_WG[5] = {128, 128, 256, 256, 256}, ENS_SHAPE_V[5] = {0, 1, 0, 1, 2};
+
+static bool ens_shape_ok(dns_ensemble *e, int c, int wg, int v) {
+  for (dns_handle *h : e->h)
+    if (!dnsk::ens_plane_variant_ok(ens_plane_args(h, 1), c, v, wg)) return false;
+  return true;
+}
+
+extern "C" int dns_ensemble_set_tiles(dns_ensemble *e, const int *var, const int *wg) {
+  if (!e || !var || !wg) return fail(DNS_ERR_ARG, "dns_ensemble_set_tiles: null argument");
+  for (int c = 0; c < 5; c++)
+    if (!ens_shape_ok(e, c, wg[c], var[c]))
+      return fail(DNS_ERR_ARG, "dns_ensemble_set_tiles: class %d: variant %d in %d-thread workgroups is not "
+                  "available for every group", c, var[c], wg[c]);
+  if (var[5] < 0 || var[5] >= dnsk::ENS_NVAR)
+    return fail(DNS_ERR_ARG, "dns_ensemble_set_tiles: column stage: no variant %d", var[5]);
+  ens_drop_graph(e);
+  for (int c = 0; c < 6; c++) e->var[c] = var[c];
+  for (int c = 0; c < 5; c++) e->wg[c] = wg[c];
+  return DNS_OK;
+}
+
+extern "C" int dns_ensemble_get_tiles(dns_ensemble *e, int *var, int *wg) {
+  if (!e || !var || !wg) return fail(DNS_ERR_ARG, "dns_ensemble_get_tiles: null argument");
+  for (int c = 0; c < 6; c++) var[c] = e->var[c];
+  for (int c = 0; c < 5; c++) wg[c] = e->wg[c];
+  return DNS_OK;
+}
+
+// Times every tile shape of every grouped class on the whole launch (all groups, all members) and keeps
+// the fastest per class: five shapes per plane class (kRK = 2 arguments: no CFL maxima), three column-pass
+// variants.  The fields the column pass overwrites are saved and put back, so the ensemble's state is
+// unchanged.
+extern "C" int dns_ensemble_autotune(dns_ensemble *e, int iters, int *var, int *wg, double *ms) {
+  if (!e) return fail(DNS_ERR_ARG, "dns_ensemble_autotune: null ensemble");
+  if (iters < 1 || iters > 1000) return fail(DNS_ERR_ARG, "dns_ensemble_autotune: iters=%d not in 1..1000", iters);
+  int rc = ens_begin(e, "dns_ensemble_autotune");
+  if (rc) return rc;
+  const int ng = (int)e->h.size();
+  if (ms)
+    for (int i = 0; i < DNS_ENS_TUNE_MS; i++) ms[i] = -1.0;
+  std::vector<TuneSave> sv(ng);
+  for (int g = 0; g < ng && rc == DNS_OK; g++) rc = tune_save(e->h[g], e->stream, sv[g]);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == DNS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(DNS_ERR_HIP, "dns_ensemble_autotune: event creation failed");
+  // one warm-up launch, then iters timed ones
+  auto timed = [&](auto launch, double &t) -> int {
+    if (launch()) return fail(DNS_ERR_ARG, "dns_ensemble_autotune: a size is not instantiated");
+    hipError_t r = hipEventRecord(e0, e->stream);
+    for (int it = 0; it < iters && r == hipSuccess; it++) launch();
+    if (r == hipSuccess) r = hipEventRecord(e1, e->stream);
+    if (r == hipSuccess) r = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (r == hipSuccess) r = hipEventElapsedTime(&f, e0, e1);
+    if (r != hipSuccess) return fail(DNS_ERR_HIP, "dns_ensemble_autotune: %s", hipGetErrorString(r));
+    t = (double)f / iters;
+    return DNS_OK;
+  };
+  dnsk::EnsPlaneLaunch P;
+  memset(&P, 0, sizeof P);
+  dnsk::EnsColLaunch Cl;
+  memset(&Cl, 0, sizeof Cl);
+  P.ng = Cl.ng = ng;
+  for (int g = 0; g < ng; g++) {
+    P.a[g] = ens_plane_args(e->h[g], 2);
+    Cl.a[g] = col_args(e->h[g], 2);
+  }
+  int nvar[6] = {1, 1, 1, 1, 1, 0}, nwg[5] = {128, 128, 128, 128, 128};
+  for (int c = 0; c < 5 && rc == DNS_OK; c++) {
+    double tb = 1e300;
+    for (int k = 0; k < 5 && rc == DNS_OK; k++) {
+      const int w = ENS_SHAPE_WG[k], v = ENS_SHAPE_V[k];
+      if (!ens_shape_ok(e, c, w, v)) continue;
+      double t = -1.0;
+      rc = timed([&]() { return dnsk::launch_ens_plane(P, e->fp32, c, w, v, e->stream); }, t);
+      if (ms) ms[c * 5 + k] = t;
+      if (rc == DNS_OK && t < tb) { tb = t; nwg[c] = w; nvar[c] = v; }
+    }
+  }
+  double tb = 1e300;
+  for (int v = 0; v < dnsk::ENS_NVAR && rc == DNS_OK; v++) {
+    double t = -1.0;
+    rc = timed([&]() { return dnsk::launch_ens_column_pass(Cl, e->fp32, v, e->stream); }, t);
+    if (ms) ms[25 + v] = t;
+    if (rc == DNS_OK && t < tb) { tb = t; nvar[5] = v; }
+  }
+  for (int g = 0; g < ng; g++) tune_restore(e->h[g], e->stream, sv[g]);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc == DNS_OK) {
+    ens_drop_graph(e);
+    for (int c = 0; c < 6; c++) e->var[c] = nvar[c];
+    for (int c = 0; c < 5; c++) e->wg[c] = nwg[c];
+    if (var && wg) dns_ensemble_get_tiles(e, var, wg);
+  }
+  const int rc2 = ens_end(e);
+  return rc ? rc : rc2;
 }
 
 // ---- statistics and spectra (stats.f90, spectra.f90) ---------------------------------------------

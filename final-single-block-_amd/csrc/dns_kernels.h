@@ -217,14 +217,15 @@ int four_to_phys_scratch(void **ptr);
 bool column_pass_supported(int nyu);
 int column_pass_variants(int nyu);  // workgroup shapes of nyu's bucket (same arithmetic, same bits)
 int launch_column_pass(const ColArgs &a, int variant, hipStream_t st);  // 0, or 1 when nyu is not covered
-// ---- ensemble of several sizes: grouped launches (ensemble.hip, column_pass.hip) ----
+// ---- ensemble of several sizes: grouped launches (ens_plane*.hip, column_pass.hip, ensemble.hip) ----
 constexpr int ENS_MAXG = 4;          // size groups per ensemble
-constexpr int ENS_PLANE_NTH = 128;   // block size of the grouped plane launches
-// one grouped plane launch: group g's arguments, the transform length of this kernel class, its grid
-// (gx, gy) and its first block (off); filled by launch_ens_plane
+constexpr int ENS_PLANE_NTH = 128;   // threads of a plane body of the persistent launch
+constexpr int ENS_NVAR = 3;          // tile variants per plane class and length, per column row bucket
+// one grouped plane launch: group g's arguments; the launcher fills the transform length of the class,
+// the logical grid (gx, gy; nblk blocks) and each group's first workgroup (off)
 struct EnsPlaneLaunch {
   PlaneArgs a[ENS_MAXG];
-  int len[ENS_MAXG], gx[ENS_MAXG], gy[ENS_MAXG], off[ENS_MAXG + 1];
+  int len[ENS_MAXG], gx[ENS_MAXG], gy[ENS_MAXG], nblk[ENS_MAXG], off[ENS_MAXG + 1];
   int ng;
 };
 // the small per-substep kernels of every group
@@ -236,7 +237,8 @@ struct EnsSmall {
   int nm[ENS_MAXG], C[ENS_MAXG], gx[ENS_MAXG], off[ENS_MAXG + 1];
   int ng;
 };
-// one grouped column-pass launch (shape by row count, up to 160 rows; 512-thread workgroups)
+// one grouped column-pass launch (shape by row count, up to 160 rows, and the launch's variant; 256-thread
+// workgroups); the launcher fills nblk and off
 struct EnsColLaunch {
   ColArgs a[ENS_MAXG];
   int nblk[ENS_MAXG], off[ENS_MAXG + 1];
@@ -244,13 +246,17 @@ struct EnsColLaunch {
 };
 bool ens_plane_length_supported(int n);
 int ens_plane_rows_ok(int Mx, int Mz);
-// which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; every group's j0 = 1, nbj = nyu
-int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, hipStream_t st);
+// which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; every group's j0 = 1, nbj = nyu;
+// tile shape: wg = 128 or 256 threads per workgroup, bodies of 64 << v threads.  -1 when a group's size
+// or the shape is not supported.
+int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, int wg, int v, hipStream_t st);
+// whether the group with arguments a (j0 = 1, nbj = nyu) takes shape (wg, v) of class which
+bool ens_plane_variant_ok(const PlaneArgs &a, int which, int v, int wg);
 void launch_ens_reset(const EnsSmall &E, int kRK, hipStream_t st);
 void launch_ens_ghost(EnsSmall E, hipStream_t st);
 void launch_ens_cfl_reduce(const EnsSmall &E, hipStream_t st);
 bool ens_column_rows_supported(int nyu);
-int launch_ens_column_pass(EnsColLaunch E, bool fp32, hipStream_t st);
+int launch_ens_column_pass(EnsColLaunch E, bool fp32, int v, hipStream_t st);  // v: variant 0..2
 // meanflow_ctU after the column pass for the groups with a constant flow rate (flags[g] = 1)
 void launch_ens_meanflow_post(const EnsColLaunch &E, const int *flags, hipStream_t st);
 

@@ -1,0 +1,10 @@
+// Grouped register x-pass launches of the ensemble (ens_plane.h), FP32 planes.
+#include "ens_plane.h"
+
+namespace dnsk {
+
+int launch_ens_xpass_f32(const EnsPlaneLaunch &E, bool vpl, int wg, int v, hipStream_t st) {
+  return vpl ? launch_ens_plane_t<float, 2>(E, wg, v, st) : launch_ens_plane_t<float, 1>(E, wg, v, st);
+}
+
+}  // namespace dnsk

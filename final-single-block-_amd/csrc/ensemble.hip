@@ -1,92 +1,20 @@
-// Grouped launches of the ensemble form: channels of different sizes (one dns_handle per size, each
-// holding its same-size members) advance together, one launch per kernel class per RK3 substep for all
-// members of all sizes.  Each launch is a 1-D grid whose block ranges belong to the groups in turn; a
-// block finds its group, takes its logical index inside the group's grid, and runs the body of the
-// group's transform length (plane kernels) or row count (column pass) from plane_kernels.h /
-// column_pass.hip.  Every member runs the reference's single-channel RK3 loop (littleharsh.f90:163-233).
-//
-// Tiles: 128-thread workgroups for the plane kernels at every length (z-kernels: 32, 16, 8 columns at
-// 48, 96, 192 points; the register x-pass: 16, 8, 4 rows), so that one launch has one block size.  All
-// tile variants of a kernel compute the same bits, so a member's result equals the same channel run alone.
-#include "plane_launch.h"
+// Ensemble form: the grouped launches' dispatch and small per-substep kernels, and the persistent launch.
+// The grouped plane kernels are in ens_plane.h (instantiated per class in ens_plane_*.hip), the grouped
+// column pass in column_pass.hip.
+#include "ens_plane.h"
 #include "column_pass.h"
 #include "column_seq.h"
 
 namespace dnsk {
 
-// ---- plane kernels -------------------------------------------------------------------------------
-template <int N> struct EnsZ {   // z-kernel tile: ZTile variant with 128 threads
-  static constexpr int V = N == 48 ? 2 : N == 96 ? 0 : 1;
-  using Z = ZTile<N, V>;
-  static_assert(Z::NTH == ENS_PLANE_NTH, "one block size per launch");
-};
-template <typename R, int N> using EZinv = ZinvBody<R, N, EnsZ<N>::Z::T, ENS_PLANE_NTH, typename ZPlanT<N>::type,
-                                                    EnsZ<N>::Z::ZCH>;
-template <typename R, int N, bool VPL>
-using EZfwd = ZfwdBody<R, N, EnsZ<N>::Z::T, ENS_PLANE_NTH, VPL, typename ZPlanT<N>::type>;
-// which: 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes; body of length N
-template <typename R, int WHICH, int N> struct EnsBody;
-template <typename R, int N> struct EnsBody<R, 0, N> { using B = EZinv<R, N>; };
-// x-pass: the register x-pass body of the single-channel launches, 128 threads
-template <typename R, int N, bool VPL> using EX = XregBody<R, N, ENS_PLANE_NTH, VPL>;
-template <typename R, int N> struct EnsBody<R, 1, N> { using B = EX<R, N, false>; };
-template <typename R, int N> struct EnsBody<R, 2, N> { using B = EX<R, N, true>; };
-template <typename R, int N> struct EnsBody<R, 3, N> { using B = EZfwd<R, N, false>; };
-template <typename R, int N> struct EnsBody<R, 4, N> { using B = EZfwd<R, N, true>; };
-
-template <typename R, int WHICH> constexpr size_t ens_plane_lds() {
-  constexpr size_t a = EnsBody<R, WHICH, 48>::B::LDS, b = EnsBody<R, WHICH, 96>::B::LDS,
-                   c = EnsBody<R, WHICH, 192>::B::LDS;
-  return a > b ? (a > c ? a : c) : (b > c ? b : c);
-}
-
-__device__ __forceinline__ int ens_group(const int *off, int ng, int lb) {
-  int g = 0;
-  while (g + 1 < ng && lb >= off[g + 1]) g++;
-  return g;
-}
-
-template <typename R, int WHICH>
-__global__ void __launch_bounds__(ENS_PLANE_NTH) ens_plane_kernel(EnsPlaneLaunch E) {
-  __shared__ __align__(16) unsigned char lds[ens_plane_lds<R, WHICH>()];
-  // the z-kernels' logical order is XCD-aware over the whole launch (each XCD runs a contiguous run of
-  // logical blocks, so neighbouring tiles of a row share their edge lines in one L2)
-  const int lbg = (WHICH == 1 || WHICH == 2) ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-  const int g = ens_group(E.off, E.ng, lbg);
-  const int lb = lbg - E.off[g];
-  const PlaneArgs &a = E.a[g];
-  switch (E.len[g]) {
-    case 48: EnsBody<R, WHICH, 48>::B::run(a, lb, E.gx[g], E.gy[g], lds); break;
-    case 96: EnsBody<R, WHICH, 96>::B::run(a, lb, E.gx[g], E.gy[g], lds); break;
-    default: EnsBody<R, WHICH, 192>::B::run(a, lb, E.gx[g], E.gy[g], lds); break;
-  }
-}
+// per-class launchers (ens_plane_zinv.hip, ens_plane_xpass_f64/f32.hip, ens_plane_zfwd_f64/f32.hip)
+int launch_ens_zinv(const EnsPlaneLaunch &E, bool fp32, int wg, int v, hipStream_t st);
+int launch_ens_xpass_f64(const EnsPlaneLaunch &E, bool vpl, int wg, int v, hipStream_t st);
+int launch_ens_xpass_f32(const EnsPlaneLaunch &E, bool vpl, int wg, int v, hipStream_t st);
+int launch_ens_zfwd_f64(const EnsPlaneLaunch &E, bool vpl, int wg, int v, hipStream_t st);
+int launch_ens_zfwd_f32(const EnsPlaneLaunch &E, bool vpl, int wg, int v, hipStream_t st);
 
 bool ens_plane_length_supported(int n) { return n == 48 || n == 96 || n == 192; }
-
-// grid of group g for kernel class which (the same grids as the single-channel launchers)
-static void ens_plane_grid(const PlaneArgs &a, int which, int &gx, int &gy, int &nblk) {
-  const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
-  int T = 0, ZCH = 1, RB = 1;
-  switch (n) {
-    case 48: T = EnsZ<48>::Z::T; ZCH = EnsZ<48>::Z::ZCH; RB = EX<double, 48, false>::ROWS; break;
-    case 96: T = EnsZ<96>::Z::T; ZCH = EnsZ<96>::Z::ZCH; RB = EX<double, 96, false>::ROWS; break;
-    default: T = EnsZ<192>::Z::T; ZCH = EnsZ<192>::Z::ZCH; RB = EX<double, 192, false>::ROWS; break;
-  }
-  if (which == 0) {
-    gx = (a.Kx + T - 1) / T;
-    gy = 3;
-    nblk = gx * gy * ((a.nbj + ZCH - 1) / ZCH) * a.nm;
-  } else if (which <= 2) {
-    gx = (a.Mz + RB - 1) / RB;
-    gy = a.nbj * a.nm;
-    nblk = gx * gy;
-  } else {
-    gx = (a.Kx + T - 1) / T;
-    gy = a.nbj * a.nm;
-    nblk = gx * gy;
-  }
-}
 
 int ens_plane_rows_ok(int Mx, int Mz) {  // the register x-pass takes any Mz
   (void)Mx;
@@ -94,35 +22,23 @@ int ens_plane_rows_ok(int Mx, int Mz) {  // the register x-pass takes any Mz
   return 1;
 }
 
-int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, hipStream_t st) {
-  int total = 0;
+bool ens_plane_variant_ok(const PlaneArgs &a, int which, int v, int wg) { return ens_plane_fits(a, which, v, wg); }
+
+int launch_ens_plane(EnsPlaneLaunch E, bool fp32, int which, int wg, int v, hipStream_t st) {
   for (int g = 0; g < E.ng; g++) {
     const PlaneArgs &a = E.a[g];
-    const int n = (which == 1 || which == 2) ? a.Mx : a.Mz;
     if (!ens_plane_length_supported(a.Mx) || !ens_plane_length_supported(a.Mz) || a.Kx != a.Mx / 3 + 1 ||
         a.Nz != 2 * a.Mz / 3 || !ens_plane_rows_ok(a.Mx, a.Mz))
       return -1;
-    int nb = 0;
-    ens_plane_grid(a, which, E.gx[g], E.gy[g], nb);
-    E.len[g] = n;
-    E.off[g] = total;
-    total += nb;
   }
-  E.off[E.ng] = total;
-#define GO(R, W) ens_plane_kernel<R, W><<<total, ENS_PLANE_NTH, 0, st>>>(E)
-  if (fp32) {
-    switch (which) {
-      case 0: GO(float, 0); break; case 1: GO(float, 1); break; case 2: GO(float, 2); break;
-      case 3: GO(float, 3); break; default: GO(float, 4); break;
-    }
-  } else {
-    switch (which) {
-      case 0: GO(double, 0); break; case 1: GO(double, 1); break; case 2: GO(double, 2); break;
-      case 3: GO(double, 3); break; default: GO(double, 4); break;
-    }
+  switch (which) {
+    case 0: return launch_ens_zinv(E, fp32, wg, v, st);
+    case 1: case 2:
+      return fp32 ? launch_ens_xpass_f32(E, which == 2, wg, v, st) : launch_ens_xpass_f64(E, which == 2, wg, v, st);
+    case 3: case 4:
+      return fp32 ? launch_ens_zfwd_f32(E, which == 4, wg, v, st) : launch_ens_zfwd_f64(E, which == 4, wg, v, st);
+    default: return -1;
   }
-#undef GO
-  return 0;
 }
 
 // ---- per-substep helpers of all groups ------------------------------------------------------------
@@ -231,7 +147,6 @@ void launch_ens_cfl_reduce(const EnsSmall &E, hipStream_t st) {
 #define PERSIST_FN __attribute__((noinline))
 #endif
 constexpr int PERSIST_SPIN_LIMIT = 1 << 20;
-__host__ __device__ constexpr size_t al16(size_t x) { return (x + 15) / 16 * 16; }
 using lds_ptr = __attribute__((address_space(3))) unsigned char *;
 
 // LDS of one 128-thread plane body (the largest of every class and length)
@@ -501,15 +416,15 @@ int ens_persist_group(const PlaneArgs &a, int ctp0, PersistGroup &q) {
       a.Nz != 2 * a.Mz / 3 || persist_rows(a.nyu) == 0 || a.nyu < 8)
     return -1;
   int gy = 0, nb = 0;
-  ens_plane_grid(a, 0, q.gxz, gy, nb);
+  ens_plane_grid(a, 0, 1, q.gxz, gy, nb);
   q.nbz = nb / a.nm;
   // a z-inverse workgroup walks ZCH + 1 field rows; both bodies of a task must take the same number of
   // barriers, so every chunk of planes must be whole
-  const int zch = a.Mz == 48 ? EnsZ<48>::Z::ZCH : a.Mz == 96 ? EnsZ<96>::Z::ZCH : EnsZ<192>::Z::ZCH;
+  const int zch = EnsZT<48, 1>::ZCH;
   if (a.nbj % zch != 0) return -1;
-  ens_plane_grid(a, 1, q.gxx, gy, nb);
+  ens_plane_grid(a, 1, 1, q.gxx, gy, nb);
   q.nbx = nb / a.nm;
-  ens_plane_grid(a, 3, q.gxf, gy, nb);
+  ens_plane_grid(a, 3, 1, q.gxf, gy, nb);
   q.nbf = nb / a.nm;
   q.cshape = persist_rows(a.nyu);
   q.ncb = (a.C + 7) / 8;

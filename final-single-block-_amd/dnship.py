@@ -49,7 +49,8 @@ EXPORTS = ["dns_last_error", "dns_ygrid", "dns_create", "dns_destroy", "dns_size
            "dns_refresh_stats_pressure", "dns_four_to_phys", "dns_four_to_phys_noalloc", "dns_ensemble_create",
            "dns_ensemble_destroy",
            "dns_ensemble_step", "dns_ensemble_step_graph", "dns_ensemble_sync", "dns_ensemble_set_timing",
-           "dns_ensemble_get_timing", "dns_ensemble_set_launch", "dns_ensemble_get_launch"]
+           "dns_ensemble_get_timing", "dns_ensemble_set_launch", "dns_ensemble_get_launch",
+           "dns_ensemble_set_tiles", "dns_ensemble_get_tiles", "dns_ensemble_autotune"]
 
 
 def lib():
@@ -109,6 +110,9 @@ def lib():
     L.dns_ensemble_set_launch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.dns_ensemble_get_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]
+    L.dns_ensemble_set_tiles.argtypes = [vp, _ip, _ip]
+    L.dns_ensemble_get_tiles.argtypes = [vp, _ip, _ip]
+    L.dns_ensemble_autotune.argtypes = [vp, ctypes.c_int, _ip, _ip, _dp]
     L.dns_get_column_path.argtypes = [vp, _ip]
     _lib = L
     return L
@@ -400,6 +404,32 @@ class Ensemble:
         mode, nwg, nt = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 3)()
         _chk(lib().dns_ensemble_get_launch(self.e, ctypes.byref(mode), ctypes.byref(nwg), nt))
         return mode.value, nwg.value, list(nt)
+
+    # kernel classes of the grouped launches with tile shapes (dns_ensemble_set_tiles); the plane classes'
+    # shapes (threads per workgroup, variant) in the order of the autotuner's times
+    TILE_CLASSES = ("zinv", "xpass_u", "xpass_v", "zfwd_u", "zfwd_v", "col_pass")
+    PLANE_SHAPES = ((128, 0), (128, 1), (256, 0), (256, 1), (256, 2))
+    TUNE_MS = 28
+
+    def tiles(self):
+        """(var, wg): var[6] variant of each grouped class (TILE_CLASSES), wg[5] threads per workgroup of
+        the five plane classes."""
+        var, wg = (ctypes.c_int * 6)(), (ctypes.c_int * 5)()
+        _chk(lib().dns_ensemble_get_tiles(self.e, var, wg))
+        return list(var), list(wg)
+
+    def set_tiles(self, var, wg):
+        _chk(lib().dns_ensemble_set_tiles(self.e, (ctypes.c_int * 6)(*[int(v) for v in var]),
+                                          (ctypes.c_int * 5)(*[int(w) for w in wg])))
+
+    def autotune(self, iters=3):
+        """Times every tile shape of every grouped class on the whole launch and keeps the fastest; returns
+        (var, wg, ms) with ms[class][k], k over PLANE_SHAPES for the plane classes and over the three
+        variants for the column stage (-1: not available)."""
+        ms = np.full(self.TUNE_MS, -1.0)
+        _chk(lib().dns_ensemble_autotune(self.e, int(iters), None, None, _p(ms)))
+        var, wg = self.tiles()
+        return var, wg, [ms[5 * c:5 * c + 5].tolist() for c in range(5)] + [ms[25:28].tolist()]
 
     def timing(self):
         t = np.zeros(7)

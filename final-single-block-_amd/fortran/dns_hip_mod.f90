@@ -249,6 +249,24 @@ module dns_hip
       integer(c_int), intent(out) :: mode, nwg
       integer(c_int), intent(out) :: ntask(3)
     end function
+    ! tile shapes of the grouped launches: var(6), wg(5)
+    integer(c_int) function dns_ensemble_set_tiles(e, var, wg) bind(C, name='dns_ensemble_set_tiles')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), intent(in) :: var(6), wg(5)
+    end function
+    integer(c_int) function dns_ensemble_get_tiles(e, var, wg) bind(C, name='dns_ensemble_get_tiles')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), intent(out) :: var(6), wg(5)
+    end function
+    ! var, wg and ms may be c_null_ptr (ms: DNS_ENS_TUNE_MS = 28 doubles)
+    integer(c_int) function dns_ensemble_autotune(e, iters, var, wg, ms) bind(C, name='dns_ensemble_autotune')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: iters
+      type(c_ptr), value :: var, wg, ms
+    end function
     ! device pointers (type(c_ptr)) and a stream (c_null_ptr: the null stream); no device allocation
     integer(c_int) function dns_four_to_phys_noalloc(Nx, Nz, Mx, Mz, nplanes, d_in, d_out, stream) &
         bind(C, name='dns_four_to_phys_noalloc')

@@ -234,6 +234,24 @@ int dns_ensemble_get_launch(dns_ensemble *e, int *mode, int *nwg, int *ntask);
  * t[2] z-forward, t[3] column stage (+ CFL reduction, mean-flow correction), t[6] whole substep (ms). */
 int dns_ensemble_set_timing(dns_ensemble *e, int on);
 int dns_ensemble_get_timing(dns_ensemble *e, double *t, int *n);
+/* Tile shapes of the grouped launches (mode 0); no counterpart in the reference (its FFT and tridiagonal
+ * work has no tiles).  var[c], c = 0 z-inverse, 1/2 x-pass u/v-planes, 3/4 z-forward u/v-planes: bodies of
+ * 64 << var[c] threads (var 0..2), wg[c] threads per workgroup (128 or 256; a workgroup runs wg / (64 <<
+ * var) bodies side by side); var[5]: column stage variant (0..2: a columns-per-workgroup count or form of
+ * each group's row bucket with the same segments).  One shape per class for every group of the launch.
+ * Defaults: plane variant 1 in 128-thread workgroups, column variant 0.  Every shape gives the same bits.
+ * set rejects (DNS_ERR_ARG) a body wider than its workgroup, and several z-inverse bodies per workgroup
+ * where a group's ny - 1 is odd; it drops a captured step graph. */
+#define DNS_ENS_TUNE_MS 28
+int dns_ensemble_set_tiles(dns_ensemble *e, const int *var, const int *wg);
+int dns_ensemble_get_tiles(dns_ensemble *e, int *var, int *wg);
+/* The ensemble's autotuner: times every shape of every grouped class on the whole launch (all groups and
+ * members; plane classes with kRK = 2 arguments) and keeps the fastest per class; the fields the column
+ * stage overwrites are saved and put back.  var[6], wg[5] (both may be NULL) receive the choice as
+ * dns_ensemble_get_tiles; ms[DNS_ENS_TUNE_MS] (may be NULL) the time per launch (ms, -1 where not
+ * available) of plane class c in shape k at ms[5c + k], k = 0..4 for (wg, var) = (128, 0), (128, 1),
+ * (256, 0), (256, 1), (256, 2), and of column variant v at ms[25 + v]. */
+int dns_ensemble_autotune(dns_ensemble *e, int iters, int *var, int *wg, double *ms);
 
 /* Waits for all work on the handle's stream. */
 int dns_sync(dns_handle *h);

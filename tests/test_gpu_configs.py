@@ -173,8 +173,9 @@ def ensemble_members():
 @pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
 def test_ensemble_48_members_three_sizes(prec, launch):
-    """The 48-member ensemble of three sizes, in one grouped launch per kernel class per substep (launch 0)
-    or one persistent launch per substep over a task queue (launch 1), its RK3 step captured once in a
+    """The 48-member ensemble of three sizes, in one grouped launch per kernel class per substep with tile
+    shapes from the ensemble autotuner (launch 0) or one persistent launch per substep over a task queue
+    (launch 1), its RK3 step captured once in a
     hipGraph and replayed (3 steps): every member equals the same channel run alone (autotuned) bit for bit
     (own IC, Re, mpgx, CFL dt), and the first member of each size agrees with the oracle (FP64: <= 1e-12;
     mixed: its FP32 plane stage's error, <= 1e-6 after 3 steps)."""
@@ -198,6 +199,15 @@ def test_ensemble_48_members_three_sizes(prec, launch):
         mode, nwg, ntask = ens.launch()
         print("persistent launch: %d workgroups, %s tasks per substep" % (nwg, ntask), flush=True)
         assert mode == 1 and nwg >= 256 and min(ntask) > 0
+    else:
+        # tile shapes from the ensemble's autotuner, which puts the fields back as it found them
+        var, wg, ms = ens.autotune(iters=2)
+        print("autotuned tiles: var %s wg %s" % (var, wg), flush=True)
+        assert (var, wg) == ens.tiles()
+        for c in range(5):
+            k = dnship.Ensemble.PLANE_SHAPES.index((wg[c], var[c]))
+            assert ms[c][k] == min(t for t in ms[c] if t >= 0)
+        assert ms[5][var[5]] == min(ms[5])
     ens.step_graph(nsteps)
     ens.sync()  # raises if a task of the persistent launch waited too long
     ens.close()
@@ -241,9 +251,7 @@ def test_ensemble_grouped_eager_and_graph(prec, launch):
     plane and a group of one: 2 eager steps then 2 graph-replayed steps equal every member's single-channel
     run bit for bit, t, dt, mpgx and iter included; the handles' own downloads wait for the ensemble's
     work."""
-    groups = [(48, 48, 33, 0, [(3250.0, -3.0675e-3, 31), (2500.0, -3.5e-3, 32)]),
-              (96, 192, 65, 1, [(4000.0, -2.8e-3, 33)]),
-              (192, 96, 17, 1, [(3000.0, -3.2e-3, 34), (3500.0, -3.0e-3, 35), (2800.0, -3.3e-3, 36)])]
+    groups = ENS_GROUPS
     handles, ics = [], []
     for Mx, Mz, ny, ctp, mem in groups:
         h = dnship.DNS(Mx, Mz, ny, precision=prec, members=len(mem), flag_ctpress=ctp)
@@ -279,6 +287,52 @@ def test_ensemble_grouped_eager_and_graph(prec, launch):
         h.close()
 
 
+ENS_GROUPS = [(48, 48, 33, 0, [(3250.0, -3.0675e-3, 31), (2500.0, -3.5e-3, 32)]),
+              (96, 192, 65, 1, [(4000.0, -2.8e-3, 33)]),
+              (192, 96, 17, 1, [(3000.0, -3.2e-3, 34), (3500.0, -3.0e-3, 35), (2800.0, -3.3e-3, 36)])]
+
+
+@pytest.mark.parametrize("prec", [dnship.PREC_FP64, dnship.PREC_MIXED])
+def test_ensemble_every_tile_shape(prec):
+    """Every tile shape of the grouped launches (the five plane shapes of every plane class, the three
+    column-stage variants) gives every member the bits of its single-channel run (one step)."""
+    handles, ics, refs = [], [], []
+    for Mx, Mz, ny, ctp, mem in ENS_GROUPS:
+        h = dnship.DNS(Mx, Mz, ny, precision=prec, members=len(mem), flag_ctpress=ctp)
+        sts, rs = [], []
+        for m, (Re, mp, sd) in enumerate(mem):
+            st = dns_ic.seeded_channel(Mx, Mz, ny, seed=sd, broadband=True)
+            h.set_member_params(m, Re, mp)
+            s = dnship.DNS(Mx, Mz, ny, Re=Re, mpgx=mp, precision=prec, flag_ctpress=ctp)
+            s.upload(*st)
+            s.step(1)
+            rs.append(s.download())
+            s.close()
+            sts.append(st)
+        handles.append(h)
+        ics.append(sts)
+        refs.append(rs)
+    ens = dnship.Ensemble(handles)
+    assert ens.tiles() == ([1, 1, 1, 1, 1, 0], [128] * 5)    # the defaults
+    for k, (wg, v) in enumerate(dnship.Ensemble.PLANE_SHAPES):
+        var = [v] * 5 + [k % 3]
+        print("case: ensemble tiles var %s wg %d prec %d" % (var, wg, prec), flush=True)
+        ens.set_tiles(var, [wg] * 5)
+        assert ens.tiles() == (var, [wg] * 5)
+        for h, sts in zip(handles, ics):
+            for m, st in enumerate(sts):
+                h.upload(*st, member=m)
+        ens.step(1)
+        ens.sync()
+        for (Mx, Mz, ny, _, mem), h, rs in zip(ENS_GROUPS, handles, refs):
+            for m, ref in enumerate(rs):
+                for x, y in zip(h.download(member=m), ref):
+                    assert np.array_equal(x, y), (k, Mx, Mz, ny, m)
+    ens.close()
+    for h in handles:
+        h.close()
+
+
 def test_ensemble_rejects_unsupported_groups():
     a = dnship.DNS(48, 48, 33)
     b = dnship.DNS(576, 576, 17)                  # 576 points: not a grouped-launch length
@@ -298,6 +352,17 @@ def test_ensemble_rejects_unsupported_groups():
     with pytest.raises(dnship.DnsError):
         e.set_launch(1)
     assert e.launch()[0] == 0                     # still the grouped launches
+    # several z-inverse bodies per workgroup need whole chunks of planes, too
+    with pytest.raises(dnship.DnsError):
+        e.set_tiles([0, 1, 1, 1, 1, 0], [256, 128, 128, 128, 128])
+    e.set_tiles([1, 1, 1, 1, 1, 0], [128, 256, 128, 128, 128])
+    for var, wg in (([2, 1, 1, 1, 1, 0], [128] * 5),   # 256-thread bodies in 128-thread workgroups
+                    ([1, 3, 1, 1, 1, 0], [128] * 5),   # no variant 3
+                    ([1, 1, 1, 1, 1, 3], [128] * 5),
+                    ([1, 1, 1, 1, 1, 0], [512] * 5)):
+        with pytest.raises(dnship.DnsError):
+            e.set_tiles(var, wg)
+    assert e.tiles() == ([1, 1, 1, 1, 1, 0], [128, 256, 128, 128, 128])
     e.close()
     for h in (a, b, c, d, f):
         h.close()
