@@ -222,10 +222,10 @@ constexpr int ENS_MAXG = 4;          // size groups per ensemble
 constexpr int ENS_PLANE_NTH = 128;   // threads of a plane body of the persistent launch
 constexpr int ENS_NVAR = 3;          // tile variants per plane class and length, per column row bucket
 // one grouped plane launch: group g's arguments; the launcher fills the transform length of the class,
-// the logical grid (gx, gy; nblk blocks) and each group's first workgroup (off)
+// the logical grid (gx, gy; nblk blocks), the group's workgroups (nwg) and its first workgroup (off)
 struct EnsPlaneLaunch {
   PlaneArgs a[ENS_MAXG];
-  int len[ENS_MAXG], gx[ENS_MAXG], gy[ENS_MAXG], nblk[ENS_MAXG], off[ENS_MAXG + 1];
+  int len[ENS_MAXG], gx[ENS_MAXG], gy[ENS_MAXG], nblk[ENS_MAXG], nwg[ENS_MAXG], off[ENS_MAXG + 1];
   int ng;
 };
 // the small per-substep kernels of every group

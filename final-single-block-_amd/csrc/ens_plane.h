@@ -77,11 +77,16 @@ __device__ __forceinline__ void ens_plane_run(const PlaneArgs &a, int lw, int gx
 template <typename R, int WHICH, int WG, int V>
 __global__ void __launch_bounds__(WG) ens_plane_kernel(EnsPlaneLaunch E) {
   __shared__ __align__(16) unsigned char lds[ens_plane_lds<R, WHICH, WG, V>()];
-  // the z-kernels' workgroup order is XCD-aware over the whole launch (each XCD runs a contiguous run of
-  // workgroups, so neighbouring tiles of a row share their edge lines in one L2)
-  const int wg = (WHICH == 1 || WHICH == 2) ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-  const int g = ens_group(E.off, E.ng, wg);
-  const int lw = wg - E.off[g];
+  const int g = ens_group(E.off, E.ng, blockIdx.x);
+  int lw = blockIdx.x - E.off[g];
+  if constexpr (WHICH == 0 || WHICH >= 3) {
+    // z-kernels: each group's workgroups start at a multiple of 8 (the padding workgroups leave at once),
+    // and their order inside the group is XCD-aware: each XCD runs a contiguous run of the group's
+    // workgroups (neighbouring tiles of a row share their edge lines in one L2), and every XCD gets its
+    // share of every group (the groups' tiles differ in cost)
+    if (lw >= E.nwg[g]) return;
+    lw = xcd_block(lw, E.nwg[g]);
+  }
   const PlaneArgs &a = E.a[g];
   const int gx = E.gx[g], gy = E.gy[g], nb = E.nblk[g];
   switch (E.len[g]) {
@@ -135,8 +140,9 @@ int launch_ens_plane_t(EnsPlaneLaunch E, int wg, int v, hipStream_t st) {
     ens_plane_grid(E.a[g], WHICH, v, E.gx[g], E.gy[g], nb);
     E.len[g] = (WHICH == 1 || WHICH == 2) ? E.a[g].Mx : E.a[g].Mz;
     E.nblk[g] = nb;
+    E.nwg[g] = (nb + sub - 1) / sub;
     E.off[g] = total;
-    total += (nb + sub - 1) / sub;
+    total += (WHICH == 1 || WHICH == 2) ? E.nwg[g] : (E.nwg[g] + 7) / 8 * 8;
   }
   E.off[E.ng] = total;
   if (wg == 128 && v == 0) ens_plane_kernel<R, WHICH, 128, 0><<<total, 128, 0, st>>>(E);
