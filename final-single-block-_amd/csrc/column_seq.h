@@ -159,23 +159,45 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
       auto at = [&](int j) { return (size_t)j * C + cl; };
       double2 um = uf[at(cu(jb - 1))], uc = uf[at(cu(jb))];
       double2 fm = fl[at(cv(jb - 1))];
+      // the global loads of a row are issued one row ahead, before the previous row's arithmetic and
+      // store, so a wavefront has two rows of loads in flight instead of one
+      struct RowIn {
+        double2 pc, up, fc, nu, nx;
+        double g1, g2, g3, hu;
+      };
+      auto load_row = [&](int r) {
+        RowIn q;
+        const int j = jb + r, ju = cp(j);
+        const size_t oq = at(ju - 1);
+        if constexpr (F == 0) q.pc = a.p[oq];
+        q.up = uf[at(cu(j + 1))];
+        q.fc = fl[at(cv(j))];
+        q.nu = nuo[oq];
+        q.nx = nxz[oq];
+        q.g1 = D2U(1, ju);
+        q.g2 = D2U(2, ju);
+        q.g3 = D2U(3, ju);
+        q.hu = a.dthdyu[ju];
+        return q;
+      };
+      RowIn nxt = load_row(0);
 #pragma unroll
       for (int r = 0; r < M; r++) {
+        const RowIn q = nxt;
+        if (r + 1 < M) nxt = load_row(r + 1);
         const int j = jb + r;
         const bool ur = j <= nyu;
         const int ju = cp(j);
         double2 pc;
         if constexpr (F == 0) {
-          pc = a.p[at(ju - 1)];
+          pc = q.pc;
           pst[r * NTH + tid] = pc;  // p row cp(j)
         } else {
           pc = pst[r * NTH + tid];
         }
-        const double2 up = uf[at(cu(j + 1))];
-        const double2 fc = fl[at(cv(j))];
+        const double2 up = q.up, fc = q.fc, nu = q.nu, nx = q.nx;
         const size_t oq = at(ju - 1);
-        const double2 nu = nuo[oq], nx = nxz[oq];
-        const double g1 = D2U(1, ju), g2 = D2U(2, ju), g3 = D2U(3, ju), hu = a.dthdyu[ju];
+        const double g1 = q.g1, g2 = q.g2, g3 = q.g3, hu = q.hu;
         const double dd2 = k2 + g2;
         const double2 Lf = cadd(cadd(rmul(g1, um), rmul(dd2, uc)), rmul(g3, up));
         double2 rf = cadd(cadd(rmul(C1, Lf), rmul(C2, nu)), imul(c3, pc));

@@ -1287,7 +1287,10 @@ extern "C" int dns_autotune(dns_handle *h, int iters, int *chosen, double *ms) {
       if (e != hipSuccess) { rc = fail(DNS_ERR_HIP, "dns_autotune: %s", hipGetErrorString(e)); break; }
       const double tv = (double)t / iters;
       if (ms) ms[3 * c + v] = tv;
-      if (tv < tbest) { tbest = tv; best = v; }
+      // another variant replaces the first only when it is at least 2 % faster: closer timings are within
+      // the run-to-run spread, and the one-pass column shapes timed on their own ranked within that spread
+      // in the opposite order to their times inside the step
+      if (v == 0 ? tv < tbest : tv < 0.98 * tbest) { tbest = tv; best = v; }
     }
     if ((c >= 5 && c < 8 && h->colpass) || (c == 8 && !h->colpass)) best = h->var[c];  // path not in use
     h->var[c] = best;

@@ -32,8 +32,9 @@ template <typename R, int N, int V> struct EnsBodyV<R, 0, N, V> {
   using Z = EnsZT<N, V>;
   using B = ZinvBody<R, N, Z::T, Z::NTH, typename ZPlanT<N>::type, Z::ZCH>;
 };
-template <typename R, int N, int V> struct EnsBodyV<R, 1, N, V> { using B = XregBody<R, N, 64 << V, false>; };
-template <typename R, int N, int V> struct EnsBodyV<R, 2, N, V> { using B = XregBody<R, N, 64 << V, true>; };
+// (the grouped launches' x-pass bodies take their twiddles from LDS, as the single-channel kernel does)
+template <typename R, int N, int V> struct EnsBodyV<R, 1, N, V> { using B = XregBody<R, N, 64 << V, false, true>; };
+template <typename R, int N, int V> struct EnsBodyV<R, 2, N, V> { using B = XregBody<R, N, 64 << V, true, true>; };
 template <typename R, int N, int V> struct EnsBodyV<R, 3, N, V> {
   using Z = EnsZT<N, V>;
   using B = ZfwdBody<R, N, Z::T, Z::NTH, false, typename ZPlanT<N>::type>;
@@ -42,8 +43,11 @@ template <typename R, int N, int V> struct EnsBodyV<R, 4, N, V> {
   using Z = EnsZT<N, V>;
   using B = ZfwdBody<R, N, Z::T, Z::NTH, true, typename ZPlanT<N>::type>;
 };
-// the 128-thread variant (the persistent launch's plane bodies)
+// the 128-thread variant (the persistent launch's plane bodies; its x-pass bodies read the twiddles from
+// global memory and hold no barrier of their own)
 template <typename R, int WHICH, int N> struct EnsBody { using B = typename EnsBodyV<R, WHICH, N, 1>::B; };
+template <typename R, int N> struct EnsBody<R, 1, N> { using B = XregBody<R, N, 128, false, false>; };
+template <typename R, int N> struct EnsBody<R, 2, N> { using B = XregBody<R, N, 128, true, false>; };
 
 // LDS of a WG-thread workgroup of variant V: WG / NTH bodies of the largest length
 template <typename R, int WHICH, int WG, int V> constexpr size_t ens_plane_lds() {

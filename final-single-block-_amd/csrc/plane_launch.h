@@ -61,7 +61,7 @@ inline void zfwd_v(const PlaneArgs &a, int nw, hipStream_t st) {
 template <typename R, int MX, int V, bool VPL>
 inline void xreg_v(const PlaneArgs &a, int nw, hipStream_t st) {
   constexpr int NTH = V == 0 ? 256 : V == 1 ? 128 : 64;
-  using B = XregBody<R, MX, NTH, VPL>;
+  using B = XregBody<R, MX, NTH, VPL, true>;
   xreg_kernel<R, MX, NTH, VPL><<<dim3((a.Mz + B::RB - 1) / B::RB, nw * a.nm), NTH, 0, st>>>(a);
 }
 

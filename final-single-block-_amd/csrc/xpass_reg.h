@@ -70,7 +70,7 @@ __device__ __forceinline__ float lane_read(float v, int addr) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
-template <typename R, int MX, int NTH_, bool VPL>
+template <typename R, int MX, int NTH_, bool VPL, bool TWL = false>
 struct XregBody {
   static constexpr int NTH = NTH_;
   static constexpr int H = MX / 2;
@@ -82,15 +82,26 @@ struct XregBody {
   static constexpr int ROWS = RB;
   static constexpr int LOGL = L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : L == 16 ? 4 : L == 32 ? 5 : 6;
   static constexpr int NSTG = LOGL - 1;         // butterfly stages with twiddles (half-size h >= 2)
-  static constexpr int NSTASH = VPL ? 0 : RR;   // u-planes keep u's physical values in LDS meanwhile
-  static constexpr size_t LDS = sizeof(cx<R>) * NSTASH * NTH + sizeof(double) * (NTH / 64);
+  // TWB: the twiddles of a transform are read into registers before its first arithmetic, the stage
+  // twiddles come from the LDS table, and the stashed field is reloaded (FP64; in FP32, where the kernel
+  // holds more wavefronts per SIMD, reading them where they are used was faster)
+  static constexpr bool TWB = TWL && sizeof(R) == 8;
+  // u-planes keep u's physical values in LDS meanwhile; with TWB the v-planes keep v's there too, and
+  // both reload them where they are next used instead of holding them in registers (fewer live registers
+  // while the next field is loaded and transformed)
+  static constexpr int NSTASH = (VPL && !TWB) ? 0 : RR;
   static constexpr int Kx = MX / 3 + 1;
+  // TWL: the pre/post-processing twiddles (H entries) and the inverse's/forward's twiddles ((RR - 1) L
+  // entries) are copied to LDS at the start, so that their reads inside the transforms do not wait on
+  // the vector memory counter behind the row's stores
+  static constexpr int NTW = TWL ? H + (RR - 1) * L : 0;
+  static constexpr size_t LDS = sizeof(cx<R>) * (NSTASH * NTH + NTW) + sizeof(double) * (NTH / 64);
   static constexpr int KOUT = (Kx + L - 1) / L;  // registers holding outputs k < Kx
   __device__ __forceinline__ static void run(const PlaneArgs &a_, int lb, int gx, int gy, unsigned char *lds);
 };
 
-template <typename R, int MX, int NTH_, bool VPL>
-__device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs &a_, int lb, int gx, int gy,
+template <typename R, int MX, int NTH_, bool VPL, bool TWL>
+__device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const PlaneArgs &a_, int lb, int gx, int gy,
                                                                unsigned char *lds) {
   (void)gy;
   const int bx = lb % gx, by = lb / gx;
@@ -114,16 +125,37 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   const cx<R> *twh = reinterpret_cast<const cx<R> *>(a.twh);  // e^{-2 pi i m/H}, then twr
   const cx<R> *twr = twh + H;                                  // twr[(m1-1) L + t] = e^{-2 pi i t m1/H}
   cx<R> *stash = reinterpret_cast<cx<R> *>(lds);
-  double *red = reinterpret_cast<double *>(lds + sizeof(cx<R>) * NSTASH * NTH);
+  double *red = reinterpret_cast<double *>(lds + sizeof(cx<R>) * (NSTASH * NTH + NTW));
+  if constexpr (TWL) {
+    cx<R> *ltw = stash + NSTASH * NTH;
+    for (int i = ltid; i < H; i += NTH) ltw[i] = twx[i];
+    for (int i = ltid; i < (RR - 1) * L; i += NTH) ltw[H + i] = twr[i];
+    __syncthreads();
+    twx = ltw;
+    twr = ltw + H;
+  }
 
   // butterfly twiddles of this lane: stage s (half-size h = 2 << s), e^{-2 pi i (t mod h)/(2h)} on the
   // upper lane of each pair, 1 on the lower
+  // (with TWL they are read from the LDS copy of twx at each stage instead: twh[m] = twx[2 m], the same
+  // bits, since both tables hold the rounded e^{-2 pi i a} of the same a)
   cx<R> ws[NSTG > 0 ? NSTG : 1];
 #pragma unroll
   for (int s = 0; s < NSTG; s++) {
     const int h = 2 << s;
     ws[s] = (t & h) ? twh[(t & (h - 1)) * (H / (2 * h))] : mk<R>(1, 0);
   }
+  auto stage_tw = [&](auto hc) -> cx<R> {
+    constexpr int h = decltype(hc)::value;
+    constexpr int s = (h == 2 ? 0 : h == 4 ? 1 : h == 8 ? 2 : h == 16 ? 3 : 4);
+    if constexpr (TWB) {
+      cx<R> w = twx[(t & (h - 1)) * (H / h)];
+      if (!(t & h)) w = mk<R>(1, 0);
+      return w;
+    } else {
+      return ws[s];
+    }
+  };
 
   // lane (L - t) mod L of this lane's group, as a ds_bpermute address: it holds the Hermitian partners
   const int paddr = ((lane & ~(L - 1)) | ((L - t) & (L - 1))) << 2;
@@ -154,60 +186,85 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
         v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
       }
     } else {
+      cx<R> xa[RR], xb[RR], e[RR];
 #pragma unroll
       for (int n1 = 0; n1 < RR; n1++) {
         const int n = L * n1 + t;
-        cx<R> xa = mk<R>(0, 0), xb = mk<R>(0, 0);
-        if (n < Kx) xa = row[n];
-        if (n == 0) xa.y = 0;
-        if (n > 0 && H - n < Kx) xb = row[H - n];
-        const cx<R> sum = cadd(xa, cconj(xb)), dif = csub(xa, cconj(xb));
-        const cx<R> e = cconj(twx[n]);  // e^{+2 pi i n/MX}
-        v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e)));
+        xa[n1] = mk<R>(0, 0);
+        xb[n1] = mk<R>(0, 0);
+        if (n < Kx) xa[n1] = row[n];
+        if (n > 0 && H - n < Kx) xb[n1] = row[H - n];
+      }
+#pragma unroll
+      for (int n1 = 0; n1 < RR; n1++) e[n1] = cconj(twx[L * n1 + t]);  // e^{+2 pi i n/MX}
+      if constexpr (TWB) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int n1 = 0; n1 < RR; n1++) {
+        const int n = L * n1 + t;
+        if (n == 0) xa[n1].y = 0;
+        const cx<R> sum = cadd(xa[n1], cconj(xb[n1])), dif = csub(xa[n1], cconj(xb[n1]));
+        v[n1] = cadd(sum, cmul_si<1>(cmul(dif, e[n1])));
       }
     }
   };
   // inverse transform: in-register DFT, twiddles, cross-lane DIF; lane t ends with z[m1 + RR brev(t)]
   auto inverse = [&](cx<R> *v) {
     if constexpr (RR > 1) {
+      cx<R> tw[RR - 1];  // with TWB read before the DFT, which hides their latency
+      if constexpr (TWB) {
+#pragma unroll
+        for (int m1 = 1; m1 < RR; m1++) tw[m1 - 1] = twr[(m1 - 1) * L + t];
+        __builtin_amdgcn_sched_barrier(0);
+      }
       dft<R, RR, +1>::run(v);
 #pragma unroll
-      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmulc(v[m1], twr[(m1 - 1) * L + t]);
+      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmulc(v[m1], TWB ? tw[m1 - 1] : twr[(m1 - 1) * L + t]);
     }
     sfor<0, LOGL>([&](auto si) {
       constexpr int h = L >> (decltype(si)::value + 1);
       const R sg = (t & h) ? (R)-1 : (R)1;
+      // every exchange of the stage is issued before the first is used, so the LDS unit's latency is
+      // paid once per stage rather than once per few values
+      cx<R> pv[RR], w = mk<R>(1, 0);
+      if constexpr (h >= 2) w = stage_tw(std::integral_constant<int, h>{});
+#pragma unroll
+      for (int i = 0; i < RR; i++) pv[i] = lane_xor_c<h>(v[i]);
+      if constexpr (h >= 4) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < RR; i++) {
-        const cx<R> p = lane_xor_c<h>(v[i]);
+        const cx<R> p = pv[i];
         cx<R> d = mk<R>(fma(sg, v[i].x, p.x), fma(sg, v[i].y, p.y));  // lower: v + p, upper: p - v
-        if constexpr (h >= 2) {
-          constexpr int s = (h == 2 ? 0 : h == 4 ? 1 : h == 8 ? 2 : h == 16 ? 3 : 4);
-          d = cmulc(d, ws[s]);
-        }
+        if constexpr (h >= 2) d = cmulc(d, w);
         v[i] = d;
       }
     });
   };
   // forward transform: cross-lane DIT, twiddles, in-register DFT; lane t ends with Z[L k1 + t]
   auto forward = [&](cx<R> *v) {
+    cx<R> tw[RR > 1 ? RR - 1 : 1];  // with TWB read before the cross-lane stages, which hide their latency
+    if constexpr (TWB) {
+#pragma unroll
+      for (int m1 = 1; m1 < RR; m1++) tw[m1 - 1] = twr[(m1 - 1) * L + t];
+      __builtin_amdgcn_sched_barrier(0);
+    }
     sfor<0, LOGL>([&](auto si) {
       constexpr int h = 1 << decltype(si)::value;
       const R sg = (t & h) ? (R)-1 : (R)1;
+      cx<R> pv[RR], w = mk<R>(1, 0);
+      if constexpr (h >= 2) w = stage_tw(std::integral_constant<int, h>{});
 #pragma unroll
       for (int i = 0; i < RR; i++) {
-        cx<R> m = v[i];
-        if constexpr (h >= 2) {
-          constexpr int s = (h == 2 ? 0 : h == 4 ? 1 : h == 8 ? 2 : h == 16 ? 3 : 4);
-          m = cmul(m, ws[s]);
-        }
-        const cx<R> p = lane_xor_c<h>(m);
-        v[i] = mk<R>(fma(sg, m.x, p.x), fma(sg, m.y, p.y));  // lower: m + p, upper: p - m
+        if constexpr (h >= 2) v[i] = cmul(v[i], w);
+        pv[i] = lane_xor_c<h>(v[i]);
       }
+      if constexpr (h >= 4) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < RR; i++)
+        v[i] = mk<R>(fma(sg, v[i].x, pv[i].x), fma(sg, v[i].y, pv[i].y));  // lower: m + p, upper: p - m
     });
     if constexpr (RR > 1) {
 #pragma unroll
-      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmul(v[m1], twr[(m1 - 1) * L + t]);
+      for (int m1 = 1; m1 < RR; m1++) v[m1] = cmul(v[m1], TWB ? tw[m1 - 1] : twr[(m1 - 1) * L + t]);
       dft<R, RR, -1>::run(v);
     }
   };
@@ -215,17 +272,23 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
   const R sc = (R)(0.5 / (double)MX);
   auto store = [&](int p, const cx<R> *v) {
     cx<R> *out = P + (size_t)p * MZ * Kp;
+    cx<R> qv[KOUT], e[KOUT];  // every partner exchange (and with TWB twiddle) is read before the first use
 #pragma unroll
     for (int k1 = 0; k1 < KOUT; k1++) {
-      cx<R> q;
-      q.x = lane_read(v[RR - 1 - k1].x, paddr);
-      q.y = lane_read(v[RR - 1 - k1].y, paddr);
+      qv[k1].x = lane_read(v[RR - 1 - k1].x, paddr);
+      qv[k1].y = lane_read(v[RR - 1 - k1].y, paddr);
+      if constexpr (TWB) e[k1] = twx[L * k1 + t];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k1 = 0; k1 < KOUT; k1++) {
+      cx<R> q = qv[k1];
       if (t == 0) q = v[(RR - k1) % RR];  // Z[H - L k1] is this lane's own
       const int k = L * k1 + t;
       if (k < Kx && zok) {
         const cx<R> zk = v[k1], zm = cconj(q);
         const cx<R> sum = cadd(zk, zm), dif = csub(zk, zm);
-        cx<R> x = csub(sum, cmul_si<1>(cmul(dif, twx[k])));
+        cx<R> x = csub(sum, cmul_si<1>(cmul(dif, TWB ? e[k1] : twx[k])));
         x = cscale(x, sc);
         if (k == 0) x.y = 0;
         out[k] = x;
@@ -250,30 +313,68 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
     sq(A);
     forward(A);
     store(2, A);
-    load_inverse(0, A);  // u, kept in LDS (each lane its own slots)
-    inverse(A);
+    if constexpr (TWB) {
+      // one field row in registers at a time: w waits in LDS while ww is transformed, then u w takes its
+      // place there while uu is transformed
+      load_inverse(2, A);  // w
+      inverse(A);
 #pragma unroll
-    for (int i = 0; i < RR; i++) {
-      if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
-      stash[i * NTH + ltid] = A[i];
+      for (int i = 0; i < RR; i++) {
+        if (cfl) m3 = fmax(m3, fmax(fabs(A[i].x), fabs(A[i].y)));
+        stash[i * NTH + ltid] = A[i];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      sq(A);  // ww
+      forward(A);
+      store(3, A);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      load_inverse(0, A);  // u
+      inverse(A);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < RR; i++) {
+        if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
+        const cx<R> w = stash[i * NTH + ltid];
+        stash[i * NTH + ltid] = mk<R>(A[i].x * w.x, A[i].y * w.y);  // uw
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      sq(A);  // uu
+      forward(A);
+      store(0, A);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + ltid];
+      forward(A);
+      store(1, A);
+    } else {
+      load_inverse(0, A);  // u, kept in LDS (each lane its own slots)
+      inverse(A);
+#pragma unroll
+      for (int i = 0; i < RR; i++) {
+        if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
+        stash[i * NTH + ltid] = A[i];
+      }
+      load_inverse(2, B);  // w
+      inverse(B);
+#pragma unroll
+      for (int i = 0; i < RR; i++) {
+        if (cfl) m3 = fmax(m3, fmax(fabs(B[i].x), fabs(B[i].y)));
+        A[i] = mk<R>(A[i].x * B[i].x, A[i].y * B[i].y);  // uw
+      }
+      forward(A);
+      store(1, A);
+      sq(B);  // ww
+      forward(B);
+      store(3, B);
+#pragma unroll
+      for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + ltid];
+      sq(A);  // uu
+      forward(A);
+      store(0, A);
     }
-    load_inverse(2, B);  // w
-    inverse(B);
-#pragma unroll
-    for (int i = 0; i < RR; i++) {
-      if (cfl) m3 = fmax(m3, fmax(fabs(B[i].x), fabs(B[i].y)));
-      A[i] = mk<R>(A[i].x * B[i].x, A[i].y * B[i].y);  // uw
-    }
-    forward(A);
-    store(1, A);
-    sq(B);  // ww
-    forward(B);
-    store(3, B);
-#pragma unroll
-    for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + ltid];
-    sq(A);  // uu
-    forward(A);
-    store(0, A);
     if (a.kRK == 1) {  // uniform over the workgroup (m1 = m3 = 0 on planes past nyu - 1)
       const double b1 = block_max_sub<NTH>((double)m1, red, ltid);
       const double b3 = block_max_sub<NTH>((double)m3, red, ltid);
@@ -282,6 +383,23 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
         atomic_max_nonneg(&a.cfl[slot], b1);
         atomic_max_nonneg(&a.cfl[CFL_SLOTS + slot], b3);
       }
+    }
+  } else if constexpr (TWB) {
+    load_inverse(1, A);  // v, kept in LDS
+    inverse(A);
+#pragma unroll
+    for (int i = 0; i < RR; i++) stash[i * NTH + ltid] = A[i];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {  // u: uv, then w: wv
+      asm volatile("" ::: "memory");
+      load_inverse(f == 0 ? 0 : 2, B);
+      inverse(B);
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int i = 0; i < RR; i++) A[i] = stash[i * NTH + ltid];
+      times(B, A);
+      forward(B);
+      store(f, B);
     }
   } else {
     load_inverse(1, A);  // v
@@ -301,7 +419,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL>::run(const PlaneArgs 
 
 template <typename R, int MX, int NTH, bool VPL, int WPE = 1>
 __global__ void __launch_bounds__(NTH, WPE) xreg_kernel(PlaneArgs a) {
-  using B = XregBody<R, MX, NTH, VPL>;
+  using B = XregBody<R, MX, NTH, VPL, true>;
   __shared__ __align__(16) unsigned char lds[B::LDS];
   B::run(a, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, lds);
 }
