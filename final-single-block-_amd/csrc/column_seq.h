@@ -24,6 +24,8 @@ struct SeqLds {
   double red[NTH / 64 > 0 ? NTH / 64 : 1];
   double u1q[P * M];       // column 0's u1 for the flow rate
   double2 pst[M * NTH];    // the segment's p rows
+  double yc[4][P * M];     // y coefficients of rows j = 1..P M (index j - 1): dthdyv(cq j), dthdyu(cp j),
+                           // dthdyv(cp j - 1), dthdyv(min(cp j, nyu - 1))
 };
 
 template <int NC, int P, int M, typename NT>
@@ -84,6 +86,16 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
   auto cp = [&](int j) { return j < 1 ? 1 : j > nyu ? nyu : j; };
   auto cq = [&](int j) { return j < 1 ? 1 : j > nyv ? nyv : j; };
   const double2 z2 = d2(0.0, 0.0);
+  // The y coefficients that phases V and P and the final row loop read, copied to LDS before the first
+  // barrier (in phase U's solve): read from global memory in the final loop, each row's read waited
+  // behind the previous row's stores.
+  for (int i = tid; i < P * M; i += NTH) {
+    const int j = 1 + i, jp = cp(j);
+    L.yc[0][i] = a.dthdyv[cq(j)];
+    L.yc[1][i] = a.dthdyu[jp];
+    L.yc[2][i] = a.dthdyv[jp - 1];
+    L.yc[3][i] = a.dthdyv[jp < nyu ? jp : nyu - 1];
+  }
 
   // ---- phase U: u-grid rows.  With SPLITU (8-row segments) one field's segment at a time, u1 then u3
   // (the same matrix: eliminating one right-hand side at a time gives the same bits as both at once, and
@@ -253,7 +265,7 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
       const double2 vvn = vvc[at(cu(j + 1))];
       const size_t ov = at(jv - 1);
       const double2 nu2 = Nu2[ov], nx2 = Nxz2[ov];
-      const double e1 = D2V(1, jv), e2 = D2V(2, jv), e3 = D2V(3, jv), hv = a.dthdyv[jv];
+      const double e1 = D2V(1, jv), e2 = D2V(2, jv), e3 = D2V(3, jv), hv = L.yc[0][j - 1];
       const double2 L2 = cadd(cadd(rmul(e1, u2m), rmul(k2 + e2, u2c)), rmul(e3, u2p));
       const double C3 = -((a.dRK * 1.0) * hv);
       double2 r2 = cadd(cadd(rmul(C1, L2), rmul(C2, nu2)), rmul(C3, csub(pn, pc)));
@@ -286,7 +298,7 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
     for (int r = 0; r < M; r++) {
       const int j = jb + r;
       const int jp = cp(j);
-      const double hu = a.dthdyu[jp], hvl = a.dthdyv[jp - 1], hvu = a.dthdyv[jp < nyu ? jp : nyu - 1];
+      const double hu = L.yc[1][j - 1], hvl = L.yc[2][j - 1], hvu = L.yc[3][j - 1];
       const bool pr = j <= nyu;
       const double2 dv = cadd(cadd(imul(m.kx, su.d[0][r]), imul(m.kz, su.d[1][r])),
                               rmul(1.0 * hu, csub(sv.d[0][r], vm)));
@@ -320,7 +332,7 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
 #pragma unroll
     for (int r = 0; r < M; r++) {
       const int j = jb + r;
-      const double hv = a.dthdyv[cq(j)];
+      const double hv = L.yc[0][j - 1];
       const double2 ps = sp.d[0][r];
       const double2 u1n = csub(su.d[0][r], imul(kxd, ps));
       const double2 u3n = csub(su.d[1][r], imul(kzd, ps));
@@ -358,7 +370,7 @@ __device__ __forceinline__ void ColumnSeqBody<NC, P, M, NT>::run(const ColArgs &
 #pragma unroll
     for (int r = 0; r < M; r++) {
       const int j = jb + r;
-      const double hu = a.dthdyu[cp(j)];
+      const double hu = L.yc[1][j - 1];  // from LDS: a global read here would wait for the stores above
       const double2 dv = cadd(cadd(imul(m.kx, su.d[0][r]), imul(m.kz, su.d[1][r])),
                               rmul(1.0 * hu, csub(sv.d[0][r], vm)));
       if (j <= nyu) dmax = fmax(dmax, hypot(dv.x, dv.y));

@@ -323,6 +323,9 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const Plane
         if (cfl) m3 = fmax(m3, fmax(fabs(A[i].x), fabs(A[i].y)));
         stash[i * NTH + ltid] = A[i];
       }
+      // the maxima are pinned here: sunk to their use at the end, they kept w's and u's physical values
+      // live through the later transforms (60 more VGPRs)
+      asm volatile("" : "+v"(m3));
       __builtin_amdgcn_sched_barrier(0);
       sq(A);  // ww
       forward(A);
@@ -339,6 +342,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const Plane
         const cx<R> w = stash[i * NTH + ltid];
         stash[i * NTH + ltid] = mk<R>(A[i].x * w.x, A[i].y * w.y);  // uw
       }
+      asm volatile("" : "+v"(m1));
       __builtin_amdgcn_sched_barrier(0);
       sq(A);  // uu
       forward(A);
@@ -357,6 +361,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const Plane
         if (cfl) m1 = fmax(m1, fmax(fabs(A[i].x), fabs(A[i].y)));
         stash[i * NTH + ltid] = A[i];
       }
+      asm volatile("" : "+v"(m1));
       load_inverse(2, B);  // w
       inverse(B);
 #pragma unroll
@@ -364,6 +369,7 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const Plane
         if (cfl) m3 = fmax(m3, fmax(fabs(B[i].x), fabs(B[i].y)));
         A[i] = mk<R>(A[i].x * B[i].x, A[i].y * B[i].y);  // uw
       }
+      asm volatile("" : "+v"(m3));
       forward(A);
       store(1, A);
       sq(B);  // ww
