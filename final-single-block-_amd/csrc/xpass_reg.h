@@ -2,7 +2,8 @@
 // ops_in_planes2, FOU3D.f90:1514-1652, rft_buff.f:62-84), register-resident, for rows whose packed
 // length H = MX/2 is RR * L with RR = 1, 3 or 9 and L = 2..64 a power of two (every supported Mx).
 //
-// One row transform per group of L lanes, RR points per lane, no LDS in the transforms:
+// One row transform per group of L lanes, RR points per lane, no LDS between the transforms' passes (the
+// single-channel and grouped launches keep their twiddle tables in LDS, TWL below):
 //   inverse (sign +1), input Z[L n1 + t] in lane t, register n1 (the row's memory order, so the loads
 //     are coalesced): RR-point DFT in registers, twiddle e^{+2 pi i t m1/H}, then an L-point DFT across
 //     the lanes as radix-2 decimation-in-frequency stages (partner lane t ^ h, values exchanged with DPP
@@ -136,14 +137,16 @@ __device__ __forceinline__ void XregBody<R, MX, NTH_, VPL, TWL>::run(const Plane
   }
 
   // butterfly twiddles of this lane: stage s (half-size h = 2 << s), e^{-2 pi i (t mod h)/(2h)} on the
-  // upper lane of each pair, 1 on the lower
-  // (with TWL they are read from the LDS copy of twx at each stage instead: twh[m] = twx[2 m], the same
+  // upper lane of each pair, 1 on the lower, held in registers
+  // (with TWB they are read from the LDS copy of twx at each stage instead: twh[m] = twx[2 m], the same
   // bits, since both tables hold the rounded e^{-2 pi i a} of the same a)
   cx<R> ws[NSTG > 0 ? NSTG : 1];
+  if constexpr (!TWB) {
 #pragma unroll
-  for (int s = 0; s < NSTG; s++) {
-    const int h = 2 << s;
-    ws[s] = (t & h) ? twh[(t & (h - 1)) * (H / (2 * h))] : mk<R>(1, 0);
+    for (int s = 0; s < NSTG; s++) {
+      const int h = 2 << s;
+      ws[s] = (t & h) ? twh[(t & (h - 1)) * (H / (2 * h))] : mk<R>(1, 0);
+    }
   }
   auto stage_tw = [&](auto hc) -> cx<R> {
     constexpr int h = decltype(hc)::value;
