@@ -276,7 +276,8 @@ int dns_get_timing(dns_handle *h, double *t, int *n);
 int dns_get_plane_variants(dns_handle *h, int *var, int *count);
 int dns_set_plane_variants(dns_handle *h, const int *var);
 /* Autotuner: times each variant of each kernel that has several (iters launches; plane kernels on the
- * first plane batch) and keeps the fastest.  chosen[DNS_NKERN] (may be NULL) receives the choice,
+ * first plane batch) and keeps the fastest; another variant replaces the first only when it is at least
+ * 2 % faster (closer timings are within the run-to-run spread).  chosen[DNS_NKERN] (may be NULL) receives the choice,
  * ms[3*DNS_NKERN] (may be NULL) the time per launch of variant v of kernel c at ms[3c+v] (-1 where not
  * timed).  Rewrites scratch only (the fields are saved and restored): call it before a step or between
  * steps.  Drops a captured graph (the next dns_rk3_step_graph captures the new choice). */
